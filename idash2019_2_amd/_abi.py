@@ -1,0 +1,111 @@
+"""ctypes declarations of the C ABI in include/idash_hip.h (libidash_hip.so).
+
+The library is the product: there is no CPU fallback. If the shared object is
+missing, importing this module raises, so a missing build fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libidash_hip.so")
+
+IHE_OK = 0
+IHE_ERR_ARG = 1
+IHE_ERR_HIP = 2
+IHE_ERR_RANGE = 3
+IHE_ERR_NOMEM = 4
+IHE_ERR_UNSUPPORTED = 5
+IHE_ERR_BUSY = 6
+
+IHE_N = 1024
+IHE_CONSTANT_BIDX = 0xFFFFFFFF
+IHE_NOISE_TABLE_LEN = 1280
+
+VARIANT_AUTO, VARIANT_VALU, VARIANT_WIDE, VARIANT_MFMA = 0, 1, 2, 3
+TIMER_CLOUD, TIMER_CLOUD_VAR, TIMER_ENCRYPT, TIMER_DECRYPT, TIMER_BATCH = range(5)
+
+# Every entry point the header declares, with (restype, argtypes).
+_vp, _u32, _i32, _u64, _sz, _int = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32,
+                                    ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int)
+_pvp = ctypes.POINTER(ctypes.c_void_p)
+
+
+class Layout(ctypes.Structure):
+    _fields_ = [("num_samples", _u32), ("num_regions", _u32), ("region_size", _u32)]
+
+
+class ModelCSR(ctypes.Structure):
+    _fields_ = [("num_outputs", _u32), ("row_ptr", _vp), ("in_bidx", _vp), ("weight", _vp)]
+
+
+class BandInfo(ctypes.Structure):
+    _fields_ = [("num_outputs", _u32), ("num_groups", _u32), ("num_tiles", _u32),
+                ("num_cols", _u32), ("nnz", _u32), ("variant", _u32), ("max_window", _u32),
+                ("device_bytes", _u64)]
+
+
+SIGNATURES = {
+    "ihe_last_error": (ctypes.c_char_p, []),
+    "ihe_device_count": (_int, [ctypes.POINTER(_int)]),
+    "ihe_ctx_create": (_int, [_int, _pvp]),
+    "ihe_ctx_destroy": (_int, [_vp]),
+    "ihe_ctx_set_stream": (_int, [_vp, _vp]),
+    "ihe_ctx_get_stream": (_int, [_vp, _pvp]),
+    "ihe_sync": (_int, [_vp]),
+    "ihe_malloc": (_int, [_vp, _sz, _pvp]),
+    "ihe_free": (_int, [_vp, _vp]),
+    "ihe_host_alloc": (_int, [_vp, _sz, _pvp]),
+    "ihe_host_free": (_int, [_vp, _vp]),
+    "ihe_memcpy_h2d": (_int, [_vp, _vp, _vp, _sz]),
+    "ihe_memcpy_d2h": (_int, [_vp, _vp, _vp, _sz]),
+    "ihe_memcpy_h2d_async": (_int, [_vp, _vp, _vp, _sz]),
+    "ihe_memcpy_d2h_async": (_int, [_vp, _vp, _vp, _sz]),
+    "ihe_memset": (_int, [_vp, _vp, _int, _sz]),
+    "ihe_timing_enable": (_int, [_vp, _int]),
+    "ihe_timing_reset": (_int, [_vp]),
+    "ihe_timing_read": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(_u32)]),
+    "ihe_band_create": (_int, [_vp, ctypes.POINTER(Layout), _u32, ctypes.POINTER(ModelCSR), _int,
+                               _pvp]),
+    "ihe_band_destroy": (_int, [_vp]),
+    "ihe_band_get_info": (_int, [_vp, ctypes.POINTER(BandInfo)]),
+    "ihe_band_plan": (_int, [ctypes.POINTER(Layout), _u32, ctypes.POINTER(ModelCSR), _int,
+                             ctypes.POINTER(BandInfo)]),
+    "ihe_cloud_score": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "ihe_key_create": (_int, [_vp, _vp, _pvp]),
+    "ihe_key_destroy": (_int, [_vp]),
+    "ihe_encrypt": (_int, [_vp, _vp, ctypes.POINTER(Layout), _u32, _vp, _u64, _vp, _vp]),
+    "ihe_decrypt": (_int, [_vp, _vp, _u32, _vp, _u32, _vp]),
+    "ihe_noise_table": (_int, [_vp, _u32]),
+}
+
+_LIB = None
+
+
+class IheError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[ihe error {code}] {msg}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libidash_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int) -> None:
+    if rc != IHE_OK:
+        msg = lib().ihe_last_error()
+        raise IheError(rc, msg.decode() if msg else "")

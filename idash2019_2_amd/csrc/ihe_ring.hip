@@ -1,0 +1,282 @@
+// Exact negacyclic products a*s mod (X^N + 1, 2^32) for encrypt and decrypt.
+//
+// The reference computes b = a*s + e (tLweSymEncryptZero, eval/idash.cpp:625)
+// and phase = b - a*s (tLwePhase, :714) through TFHE's floating-point FFT.
+// Here the key s is binary, so
+//   (a*s)[j] = sum_{k : s_k = 1} A2[j - k + N],   A2 = [-a[0..N), a[0..N)]
+// is a sum of signed coefficients. One wavefront owns one ciphertext: a is
+// staged in LDS as A2 (padded one word per J so lanes reading J-strided words
+// hit distinct banks), lane l owns the J consecutive outputs j = l*J + u and
+// slides a J-register window down A2 as k advances, so each key bit costs one
+// LDS read per lane plus, when the bit is set, J adds. The key bits are
+// kernel arguments, hence wave-uniform branches.
+#include <cstring>
+
+#include "ihe_internal.h"
+
+namespace ihe {
+
+constexpr int kWavesPerBlock = 4;
+
+template <int J>
+__device__ __forceinline__ uint32_t lds_pad(uint32_t t) {
+  if constexpr (J == 1) return t;
+  else return t + t / J;
+}
+
+template <int J>
+constexpr uint32_t lds_words() {
+  return J == 1 ? 2 * kN : 2 * kN + 2 * kN / J;
+}
+
+// acc[u] = (a*s)[jb + u] for u < J, A2 staged (padded) in lds.
+template <int J>
+__device__ __forceinline__ void negacyclic_window(const int32_t *lds, const KeyBits &key,
+                                                  uint32_t jb, uint32_t acc[J]) {
+  uint32_t win[J];
+#pragma unroll
+  for (int u = 0; u < J; ++u) {
+    acc[u] = 0;
+    win[u] = (uint32_t)lds[lds_pad<J>(jb + kN + u)];
+  }
+  // At step k the logical element u (= A2[jb + N + u - k]) sits in
+  // win[(u - k) mod J]; after the step the slot of u = J-1 receives the new
+  // logical u = 0 of step k+1.
+  for (uint32_t kb = 0; kb < kN; kb += J) {
+    const uint32_t bits = (key.w[kb / 32] >> (kb % 32)) & (J == 32 ? 0xFFFFFFFFu : ((1u << J) - 1u));
+#pragma unroll
+    for (int d = 0; d < J; ++d) {
+      if ((bits >> d) & 1u) {
+#pragma unroll
+        for (int u = 0; u < J; ++u) acc[u] += win[(u - d) & (J - 1)];
+      }
+      const uint32_t t = jb + kN - (kb + (uint32_t)d) - 1u;
+      win[(J - 1 - d) & (J - 1)] = (uint32_t)lds[lds_pad<J>(t)];
+    }
+  }
+}
+
+template <int J>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_decrypt(const int32_t *__restrict__ in,
+                                                                 uint32_t num_cts, uint32_t ns,
+                                                                 KeyBits key,
+                                                                 float *__restrict__ out) {
+  __shared__ int32_t lds_all[kWavesPerBlock][lds_words<J>()];
+  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t ct = blockIdx.x * kWavesPerBlock + wave;
+  const bool active = ct < num_cts;
+  int32_t *lds = lds_all[wave];
+  if (active) {
+    const int32_t *a = in + (size_t)ct * kCtWords;
+    for (uint32_t t = lane; t < kN; t += 64) {
+      const int32_t x = a[t];
+      lds[lds_pad<J>(t)] = (int32_t)(0u - (uint32_t)x);
+      lds[lds_pad<J>(t + kN)] = x;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const uint32_t jb = lane * J;
+  if (jb >= ns) return;
+  uint32_t acc[J];
+  negacyclic_window<J>(lds, key, jb, acc);
+  const int32_t *b = in + (size_t)ct * kCtWords + kN;
+#pragma unroll
+  for (int u = 0; u < J; ++u) {
+    const uint32_t j = jb + u;
+    if (j < ns) {
+      const int32_t phase = (int32_t)((uint32_t)b[j] - acc[u]);
+      // t32tod then float: the int32 -> float conversion rounds to nearest
+      // even; scaling by 2^-32 is exact.
+      out[(size_t)ct * ns + j] = (float)phase * 0x1p-32f;
+    }
+  }
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t r[4]) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  r[0] = c0;
+  r[1] = c1;
+  r[2] = c2;
+  r[3] = c3;
+}
+
+// |e| = smallest k with (u >> 1) < table[k]; sign from bit 0.
+__device__ __forceinline__ int32_t noise_sample(uint64_t u, const uint64_t *__restrict__ table) {
+  const uint64_t m = u >> 1;
+  uint32_t lo = 0, hi = IHE_NOISE_TABLE_LEN - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (m < table[mid]) hi = mid;
+    else lo = mid + 1;
+  }
+  return (u & 1) ? -(int32_t)lo : (int32_t)lo;
+}
+
+struct EncryptParams {
+  const int8_t *__restrict__ geno;
+  const uint64_t *__restrict__ table;
+  int32_t *__restrict__ out;
+  double *__restrict__ out_var;
+  uint32_t num_cts, num_cols, ns, R, rs;
+  uint32_t seed_lo, seed_hi;
+};
+
+// One wavefront per ciphertext c (FeatIndex):
+//   a = Philox stream 0, b = a*s + e + one-hot messages (setScore).
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_encrypt(EncryptParams P, KeyBits key) {
+  constexpr int J = 16;
+  __shared__ int32_t lds_all[kWavesPerBlock][lds_words<J>()];
+  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t ct = blockIdx.x * kWavesPerBlock + wave;
+  const bool active = ct < P.num_cts;
+  int32_t *lds = lds_all[wave];
+  const uint32_t jb = lane * J;
+  if (active) {
+    int32_t *a = P.out + (size_t)ct * kCtWords;
+#pragma unroll
+    for (int q = 0; q < J / 4; ++q) {
+      uint32_t r[4];
+      const uint32_t w0 = jb + 4 * q;
+      philox(w0 / 4, ct, 0, 0, P.seed_lo, P.seed_hi, r);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        a[w0 + t] = (int32_t)r[t];
+        lds[lds_pad<J>(w0 + t)] = (int32_t)(0u - r[t]);
+        lds[lds_pad<J>(w0 + t + kN)] = (int32_t)r[t];
+      }
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  uint32_t acc[J];
+  negacyclic_window<J>(lds, key, jb, acc);
+  int32_t *b = P.out + (size_t)ct * kCtWords + kN;
+#pragma unroll
+  for (int q = 0; q < J / 2; ++q) {
+    uint32_t r[4];
+    const uint32_t j0 = jb + 2 * q;
+    philox(j0 / 2, ct, 1, 0, P.seed_lo, P.seed_hi, r);
+    const uint64_t u0 = (uint64_t)r[0] | ((uint64_t)r[1] << 32);
+    const uint64_t u1 = (uint64_t)r[2] | ((uint64_t)r[3] << 32);
+    acc[2 * q] += (uint32_t)noise_sample(u0, P.table);
+    acc[2 * q + 1] += (uint32_t)noise_sample(u1, P.table);
+  }
+#pragma unroll
+  for (int u = 0; u < J; ++u) {
+    const uint32_t j = jb + u;
+    const uint32_t r = j / P.rs, s = j - r * P.rs;
+    uint32_t v = acc[u];
+    if (r < P.R && s < P.ns) {
+      const uint32_t bidx = ct * P.R + r;
+      if (bidx < P.num_cols) {
+        const uint32_t tag = bidx / 3, val = bidx - 3 * tag;
+        const int8_t g = P.geno[(size_t)tag * P.ns + s];
+        if (g == 0 || g == 1 || g == 2) {
+          if ((uint32_t)g == val) v += kOne;
+        } else {
+          v += val == 0 ? kNan[0] : (val == 1 ? kNan[1] : kNan[2]);
+        }
+      }
+    }
+    b[j] = (int32_t)v;
+  }
+  if (lane == 0) P.out_var[ct] = 0x1p-50;
+}
+
+}  // namespace ihe
+
+using namespace ihe;
+
+extern "C" {
+
+int ihe_key_create(ihe_ctx *ctx, const int32_t *key, ihe_key **out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!key || !out) return fail(IHE_ERR_ARG, "null argument");
+  *out = nullptr;
+  ihe_key *k = new ihe_key();
+  k->ctx = ctx;
+  std::memset(&k->bits, 0, sizeof(k->bits));
+  for (uint32_t i = 0; i < kN; ++i) {
+    if (key[i] != 0 && key[i] != 1) {
+      delete k;
+      return fail(IHE_ERR_ARG, "TLWE key must be binary");
+    }
+    if (key[i]) k->bits.w[i / 32] |= 1u << (i % 32);
+  }
+  *out = k;
+  return IHE_OK;
+}
+
+int ihe_key_destroy(ihe_key *key) {
+  delete key;
+  return IHE_OK;
+}
+
+int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint32_t num_tags,
+                const int8_t *d_geno, uint64_t seed, int32_t *d_out, double *d_out_var) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!key || key->ctx != ctx) return fail(IHE_ERR_ARG, "key does not belong to this context");
+  if (!layout) return fail(IHE_ERR_ARG, "null layout");
+  const uint32_t NS = layout->num_samples;
+  if (NS == 0 || NS > kN || layout->num_regions != kN / NS ||
+      layout->region_size != kN / layout->num_regions)
+    return fail(IHE_ERR_ARG, "layout must satisfy R = N / NS and RS = N / R (eval/idash.cpp:409-410)");
+  if (num_tags == 0) return IHE_OK;
+  if (!d_geno || !d_out || !d_out_var) return fail(IHE_ERR_ARG, "null buffer");
+  if (num_tags > 0x55555555u) return fail(IHE_ERR_ARG, "too many tags");
+  if (int rc = make_current(ctx)) return rc;
+  EncryptParams P;
+  P.geno = d_geno;
+  P.table = ctx->d_noise_table;
+  P.out = d_out;
+  P.out_var = d_out_var;
+  P.num_cols = 3 * num_tags;
+  P.R = layout->num_regions;
+  P.rs = layout->region_size;
+  P.ns = NS;
+  P.num_cts = (P.num_cols + P.R - 1) / P.R;
+  P.seed_lo = (uint32_t)seed;
+  P.seed_hi = (uint32_t)(seed >> 32);
+  const dim3 grid((P.num_cts + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+  {
+    ScopedTimer t(ctx, IHE_TIMER_ENCRYPT);
+    hipLaunchKernelGGL(k_encrypt, grid, block, 0, ctx->stream, P, key->bits);
+  }
+  IHE_HIP(hipGetLastError());
+  return IHE_OK;
+}
+
+int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const int32_t *d_in,
+                uint32_t num_cts, float *d_out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!key || key->ctx != ctx) return fail(IHE_ERR_ARG, "key does not belong to this context");
+  if (num_samples == 0 || num_samples > kN) return fail(IHE_ERR_ARG, "NUM_SAMPLES must be in [1, 1024]");
+  if (num_cts == 0) return IHE_OK;
+  if (!d_in || !d_out) return fail(IHE_ERR_ARG, "null buffer");
+  if (int rc = make_current(ctx)) return rc;
+  const dim3 grid((num_cts + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
+  ScopedTimer t(ctx, IHE_TIMER_DECRYPT);
+  // J = outputs per lane, the smallest power of two with 64 * J >= NS.
+  if (num_samples <= 64) hipLaunchKernelGGL(k_decrypt<1>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);
+  else if (num_samples <= 128) hipLaunchKernelGGL(k_decrypt<2>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);
+  else if (num_samples <= 256) hipLaunchKernelGGL(k_decrypt<4>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);
+  else if (num_samples <= 512) hipLaunchKernelGGL(k_decrypt<8>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);
+  else hipLaunchKernelGGL(k_decrypt<16>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);
+  IHE_HIP(hipGetLastError());
+  return IHE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,273 @@
+// Context, device memory, errors and timing of libidash_hip.so.
+#include <cmath>
+#include <cstring>
+
+#include "ihe_internal.h"
+
+namespace ihe {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return IHE_ERR_HIP;
+}
+
+int check_ctx(ihe_ctx *ctx) {
+  if (!ctx) return fail(IHE_ERR_ARG, "null context");
+  return IHE_OK;
+}
+
+int make_current(ihe_ctx *ctx) {
+  IHE_HIP(hipSetDevice(ctx->device));
+  return IHE_OK;
+}
+
+// P(|trunc(128 g)| <= k) = 1 - erfc((k+1) / (128 sqrt 2)), scaled by 2^63.
+// gaussian32(0, alpha) in TFHE is dtot32(alpha * g); at alpha = 2^-25 that
+// is trunc(128 g) torus units (eval/idash.cpp:20-21, :625).
+void build_noise_table(uint64_t *table, uint32_t len) {
+  const double two63 = 9223372036854775808.0;
+  for (uint32_t k = 0; k < len; ++k) {
+    const double tail = std::erfc((double)(k + 1) / (128.0 * std::sqrt(2.0))) * two63;
+    uint64_t t = (uint64_t)1 << 63;
+    if (tail >= 1.0) t -= (uint64_t)tail;
+    table[k] = t;
+  }
+  table[len - 1] = (uint64_t)1 << 63;
+}
+
+ScopedTimer::ScopedTimer(ihe_ctx *c, int s) : ctx(c), slot(s) {
+  if (!ctx->timing) return;
+  TimerSlot &t = ctx->timers[slot];
+  if (t.used == t.starts.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return;
+    if (hipEventCreate(&b) != hipSuccess) {
+      (void)hipEventDestroy(a);
+      return;
+    }
+    t.starts.push_back(a);
+    t.stops.push_back(b);
+  }
+  (void)hipEventRecord(t.starts[t.used], ctx->stream);
+  stop = t.stops[t.used];
+  t.used++;
+}
+
+ScopedTimer::~ScopedTimer() {
+  if (stop) (void)hipEventRecord(stop, ctx->stream);
+}
+
+}  // namespace ihe
+
+using namespace ihe;
+
+extern "C" {
+
+const char *ihe_last_error(void) { return g_last_error.c_str(); }
+
+int ihe_device_count(int *count) {
+  if (!count) return fail(IHE_ERR_ARG, "null count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return IHE_OK;
+}
+
+int ihe_ctx_create(int device, ihe_ctx **out) {
+  if (!out) return fail(IHE_ERR_ARG, "null output");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(IHE_ERR_HIP, "no HIP device visible");
+  if (device < 0 || device >= n) return fail(IHE_ERR_ARG, "device index out of range");
+  IHE_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  IHE_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(IHE_ERR_UNSUPPORTED, std::string("kernels are built for gfx950, device is ") +
+                                         prop.gcnArchName);
+  ihe_ctx *c = new ihe_ctx();
+  c->device = device;
+  c->num_cus = prop.multiProcessorCount;
+  hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreateWithFlags");
+  }
+  c->stream = c->own_stream;
+  uint64_t table[IHE_NOISE_TABLE_LEN];
+  build_noise_table(table, IHE_NOISE_TABLE_LEN);
+  e = hipMalloc(&c->d_noise_table, sizeof(table));
+  if (e == hipSuccess) e = hipMemcpy(c->d_noise_table, table, sizeof(table), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (c->d_noise_table) (void)hipFree(c->d_noise_table);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return hip_fail(e, "noise table upload");
+  }
+  *out = c;
+  return IHE_OK;
+}
+
+int ihe_ctx_destroy(ihe_ctx *ctx) {
+  if (!ctx) return IHE_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &t : ctx->timers) {
+    for (auto ev : t.starts) (void)hipEventDestroy(ev);
+    for (auto ev : t.stops) (void)hipEventDestroy(ev);
+  }
+  if (ctx->d_noise_table) (void)hipFree(ctx->d_noise_table);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+  return IHE_OK;
+}
+
+int ihe_ctx_set_stream(ihe_ctx *ctx, void *hip_stream) {
+  if (int rc = check_ctx(ctx)) return rc;
+  ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+  return IHE_OK;
+}
+
+int ihe_ctx_get_stream(ihe_ctx *ctx, void **hip_stream) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!hip_stream) return fail(IHE_ERR_ARG, "null output");
+  *hip_stream = ctx->stream;
+  return IHE_OK;
+}
+
+int ihe_sync(ihe_ctx *ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  IHE_HIP(hipSetDevice(ctx->device));
+  IHE_HIP(hipStreamSynchronize(ctx->stream));
+  return IHE_OK;
+}
+
+int ihe_malloc(ihe_ctx *ctx, size_t bytes, void **dptr) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!dptr) return fail(IHE_ERR_ARG, "null output");
+  *dptr = nullptr;
+  if (bytes == 0) return IHE_OK;
+  IHE_HIP(hipSetDevice(ctx->device));
+  hipError_t e = hipMalloc(dptr, bytes);
+  if (e != hipSuccess) {
+    *dptr = nullptr;
+    (void)hipGetLastError();
+    return fail(IHE_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return IHE_OK;
+}
+
+int ihe_free(ihe_ctx *ctx, void *dptr) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!dptr) return IHE_OK;
+  IHE_HIP(hipSetDevice(ctx->device));
+  IHE_HIP(hipFree(dptr));
+  return IHE_OK;
+}
+
+int ihe_host_alloc(ihe_ctx *ctx, size_t bytes, void **hptr) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!hptr) return fail(IHE_ERR_ARG, "null output");
+  *hptr = nullptr;
+  if (bytes == 0) return IHE_OK;
+  IHE_HIP(hipSetDevice(ctx->device));
+  hipError_t e = hipHostMalloc(hptr, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *hptr = nullptr;
+    return fail(IHE_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  }
+  return IHE_OK;
+}
+
+int ihe_host_free(ihe_ctx *ctx, void *hptr) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!hptr) return IHE_OK;
+  IHE_HIP(hipHostFree(hptr));
+  return IHE_OK;
+}
+
+static int copy(ihe_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind,
+                bool blocking) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (bytes == 0) return IHE_OK;
+  if (!dst || !src) return fail(IHE_ERR_ARG, "null copy pointer");
+  IHE_HIP(hipSetDevice(ctx->device));
+  IHE_HIP(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+  if (blocking) IHE_HIP(hipStreamSynchronize(ctx->stream));
+  return IHE_OK;
+}
+
+int ihe_memcpy_h2d(ihe_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  return copy(ctx, dst, src, bytes, hipMemcpyHostToDevice, true);
+}
+int ihe_memcpy_d2h(ihe_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  return copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost, true);
+}
+int ihe_memcpy_h2d_async(ihe_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  return copy(ctx, dst, src, bytes, hipMemcpyHostToDevice, false);
+}
+int ihe_memcpy_d2h_async(ihe_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  return copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost, false);
+}
+
+int ihe_memset(ihe_ctx *ctx, void *dst, int value, size_t bytes) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (bytes == 0) return IHE_OK;
+  if (!dst) return fail(IHE_ERR_ARG, "null pointer");
+  IHE_HIP(hipSetDevice(ctx->device));
+  IHE_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  return IHE_OK;
+}
+
+int ihe_timing_enable(ihe_ctx *ctx, int on) {
+  if (int rc = check_ctx(ctx)) return rc;
+  ctx->timing = on != 0;
+  return IHE_OK;
+}
+
+int ihe_timing_reset(ihe_ctx *ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  IHE_HIP(hipSetDevice(ctx->device));
+  IHE_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto &t : ctx->timers) t.used = 0;
+  return IHE_OK;
+}
+
+int ihe_timing_read(ihe_ctx *ctx, int timer, double *total_ms, uint32_t *count) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (timer < 0 || timer >= IHE_TIMER_COUNT || !total_ms || !count)
+    return fail(IHE_ERR_ARG, "bad timer query");
+  IHE_HIP(hipSetDevice(ctx->device));
+  TimerSlot &t = ctx->timers[timer];
+  double sum = 0.0;
+  for (uint32_t i = 0; i < t.used; ++i) {
+    IHE_HIP(hipEventSynchronize(t.stops[i]));
+    float ms = 0.f;
+    IHE_HIP(hipEventElapsedTime(&ms, t.starts[i], t.stops[i]));
+    sum += ms;
+  }
+  *total_ms = sum;
+  *count = t.used;
+  return IHE_OK;
+}
+
+int ihe_noise_table(uint64_t *table, uint32_t len) {
+  if (!table || len != IHE_NOISE_TABLE_LEN) return fail(IHE_ERR_ARG, "bad noise table buffer");
+  build_noise_table(table, len);
+  return IHE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,228 @@
+"""Python front end of the C ABI: device buffers, bands, keys and the three
+operations of the secure-imputation path (encrypt, cloud, decrypt).
+
+Mirrors the reference's operation split (eval/idash.h:248-254): encrypt_data
+(ph1 + ph2), cloud_compute_score and decrypt_predictions, with the data held
+in device memory between them.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, lib
+
+N = _abi.IHE_N
+CT_WORDS = 2 * N
+
+
+def layout_for(num_samples: int) -> tuple[int, int, int]:
+    """(NS, R, RS) exactly as keygen_ph1 derives them (eval/idash.cpp:409-412)."""
+    if not 1 <= num_samples <= N:
+        raise ValueError("NUM_SAMPLES must be in [1, 1024] (one ciphertext packs N slots)")
+    R = N // num_samples
+    return num_samples, R, N // R
+
+
+class DeviceArray:
+    """A device allocation with a numpy shape/dtype attached."""
+
+    def __init__(self, engine: "Engine", shape, dtype):
+        self.engine = engine
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().ihe_malloc(engine.ctx, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value or 0
+
+    def upload(self, host: np.ndarray) -> "DeviceArray":
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        if host.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        if self.nbytes:
+            check(lib().ihe_memcpy_h2d(self.engine.ctx, self.ptr, host.ctypes.data, self.nbytes))
+        return self
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().ihe_memcpy_d2h(self.engine.ctx, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def fill_zero(self) -> None:
+        if self.nbytes:
+            check(lib().ihe_memset(self.engine.ctx, self.ptr, 0, self.nbytes))
+
+    def free(self) -> None:
+        if self.ptr:
+            check(lib().ihe_free(self.engine.ctx, self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            if self.ptr and self.engine.ctx:
+                lib().ihe_free(self.engine.ctx, self.ptr)
+                self.ptr = 0
+        except Exception:
+            pass
+
+
+@dataclass
+class Band:
+    engine: "Engine"
+    handle: int
+    num_in: int
+    num_out: int
+    layout: tuple
+
+    def info(self) -> dict:
+        bi = _abi.BandInfo()
+        check(lib().ihe_band_get_info(self.handle, ctypes.byref(bi)))
+        return {f: getattr(bi, f) for f, _ in _abi.BandInfo._fields_}
+
+    def free(self) -> None:
+        if self.handle:
+            check(lib().ihe_band_destroy(self.handle))
+            self.handle = 0
+
+
+class Key:
+    def __init__(self, engine: "Engine", key: np.ndarray):
+        key = np.ascontiguousarray(key, np.int32)
+        if key.shape != (N,):
+            raise ValueError("key must have N = 1024 coefficients")
+        self.engine = engine
+        h = ctypes.c_void_p()
+        check(lib().ihe_key_create(engine.ctx, key.ctypes.data, ctypes.byref(h)))
+        self.handle = h.value
+
+    def free(self) -> None:
+        if self.handle:
+            check(lib().ihe_key_destroy(self.handle))
+            self.handle = 0
+
+
+class Engine:
+    """One context on one GPU (a HIP stream plus the library's device state)."""
+
+    def __init__(self, device: int = 0):
+        n = ctypes.c_int()
+        check(lib().ihe_device_count(ctypes.byref(n)))
+        if n.value == 0:
+            raise RuntimeError("no HIP device visible")
+        h = ctypes.c_void_p()
+        check(lib().ihe_ctx_create(device, ctypes.byref(h)))
+        self.ctx = h.value
+
+    def close(self) -> None:
+        if self.ctx:
+            check(lib().ihe_ctx_destroy(self.ctx))
+            self.ctx = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- memory
+    def empty(self, shape, dtype) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, host: np.ndarray) -> DeviceArray:
+        host = np.ascontiguousarray(host)
+        return DeviceArray(self, host.shape, host.dtype).upload(host)
+
+    def sync(self) -> None:
+        check(lib().ihe_sync(self.ctx))
+
+    # -- timing
+    def timing(self, on: bool) -> None:
+        check(lib().ihe_timing_enable(self.ctx, int(on)))
+
+    def timing_reset(self) -> None:
+        check(lib().ihe_timing_reset(self.ctx))
+
+    def timing_read(self, timer: int) -> tuple[float, int]:
+        ms = ctypes.c_double()
+        cnt = ctypes.c_uint32()
+        check(lib().ihe_timing_read(self.ctx, timer, ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    # -- cloud
+    def band(self, num_samples: int, num_in: int, row_ptr, in_bidx, weight,
+             variant: int = _abi.VARIANT_AUTO) -> Band:
+        ns, R, RS = layout_for(num_samples)
+        lay = _abi.Layout(ns, R, RS)
+        row_ptr = np.ascontiguousarray(row_ptr, np.uint32)
+        in_bidx = np.ascontiguousarray(in_bidx, np.uint32)
+        weight = np.ascontiguousarray(weight, np.int32)
+        m = _abi.ModelCSR(row_ptr.size - 1, row_ptr.ctypes.data, in_bidx.ctypes.data,
+                          weight.ctypes.data)
+        h = ctypes.c_void_p()
+        check(lib().ihe_band_create(self.ctx, ctypes.byref(lay), num_in, ctypes.byref(m), variant,
+                                    ctypes.byref(h)))
+        return Band(self, h.value, num_in, row_ptr.size - 1, (ns, R, RS))
+
+    def cloud(self, band: Band, d_in: DeviceArray, d_in_var: DeviceArray | None,
+              d_out: DeviceArray, d_out_var: DeviceArray | None) -> None:
+        """cloud_compute_score (eval/idash.cpp:763-848) on device buffers (async)."""
+        if d_out.nbytes < band.num_out * CT_WORDS * 4:
+            raise ValueError("output buffer too small")
+        if d_in.nbytes < band.num_in * CT_WORDS * 4:
+            raise ValueError("input buffer too small")
+        check(lib().ihe_cloud_score(self.ctx, band.handle, d_in.ptr,
+                                    d_in_var.ptr if d_in_var is not None else None, d_out.ptr,
+                                    d_out_var.ptr if d_out_var is not None else None))
+
+    # -- keys, encrypt, decrypt
+    def key(self, key: np.ndarray) -> Key:
+        return Key(self, key)
+
+    def encrypt(self, key: Key, num_samples: int, d_geno: DeviceArray, seed: int,
+                d_out: DeviceArray, d_out_var: DeviceArray) -> int:
+        """encrypt_data_ph1 + ph2 (eval/idash.cpp:615-679); returns the ciphertext count."""
+        ns, R, RS = layout_for(num_samples)
+        tags = d_geno.shape[0]
+        if d_geno.shape != (tags, ns) or d_geno.dtype != np.int8:
+            raise ValueError("genotypes must be int8 [tags][NUM_SAMPLES]")
+        C = (3 * tags + R - 1) // R
+        if d_out.nbytes < C * CT_WORDS * 4 or d_out_var.nbytes < C * 8:
+            raise ValueError("output buffer too small")
+        lay = _abi.Layout(ns, R, RS)
+        check(lib().ihe_encrypt(self.ctx, key.handle, ctypes.byref(lay), tags, d_geno.ptr,
+                                seed & 0xFFFFFFFFFFFFFFFF, d_out.ptr, d_out_var.ptr))
+        return C
+
+    def decrypt(self, key: Key, num_samples: int, d_in: DeviceArray, num_cts: int,
+                d_out: DeviceArray) -> None:
+        """decrypt_predictions' per-ciphertext phase and rescale (eval/idash.cpp:711-719)."""
+        if d_in.nbytes < num_cts * CT_WORDS * 4 or d_out.nbytes < num_cts * num_samples * 4:
+            raise ValueError("buffer too small")
+        check(lib().ihe_decrypt(self.ctx, key.handle, num_samples, d_in.ptr, num_cts, d_out.ptr))
+
+
+def band_plan(num_samples: int, num_in: int, row_ptr, in_bidx, weight,
+              variant: int = _abi.VARIANT_AUTO) -> dict:
+    """Runs the band compiler on the host only; raises IheError like ihe_band_create."""
+    ns, R, RS = layout_for(num_samples)
+    lay = _abi.Layout(ns, R, RS)
+    row_ptr = np.ascontiguousarray(row_ptr, np.uint32)
+    in_bidx = np.ascontiguousarray(in_bidx, np.uint32)
+    weight = np.ascontiguousarray(weight, np.int32)
+    m = _abi.ModelCSR(row_ptr.size - 1, row_ptr.ctypes.data, in_bidx.ctypes.data,
+                      weight.ctypes.data)
+    bi = _abi.BandInfo()
+    check(lib().ihe_band_plan(ctypes.byref(lay), num_in, ctypes.byref(m), variant,
+                              ctypes.byref(bi)))
+    return {f: getattr(bi, f) for f, _ in _abi.BandInfo._fields_}
+
+
+def noise_table() -> np.ndarray:
+    t = np.empty(_abi.IHE_NOISE_TABLE_LEN, np.uint64)
+    check(lib().ihe_noise_table(t.ctypes.data, t.size))
+    return t

