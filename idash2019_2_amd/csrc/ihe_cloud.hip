@@ -13,64 +13,79 @@
 //
 // Device band format. Outputs with identical column lists are grouped (the
 // three classes of a target and the consecutive targets that share the same
-// nearest-tag window). A group lists its columns as packed (ct << 10 | shift)
-// words, its outputs, and a dense [nout][ncol] weight block. Groups are cut
-// into tiles of roughly equal work; one workgroup handles one tile over one
-// slice of the 2048 coefficient words.
+// nearest-tag window), up to 12 outputs. A group lists its columns as packed
+// words (byte offset of the ciphertext, plus the rotation shift for R > 1)
+// padded to whole chunks, a chunk-major weight block and a 16-entry table of
+// its output slots and biases. Groups are cut into tiles of roughly equal
+// work, and tiles into segments whose input and output offsets fit 32 bits;
+// one launch per segment, one workgroup per (tile, slice of the 2048 words).
 //
 // Exact int32 arithmetic on the VALU: with x = hi*2^16 + lo (lo, hi read as
 // signed 16-bit after the recode hi = (x + 0x8000) >> 16), and a weight w
 // within int16,
 //   w*x = w*lo + 2^16 * w*hi   (mod 2^32)
 // so two columns share one v_dot2_i32_i16 per half: a column pair costs two
-// dot2 instructions per output word.
+// dot2 instructions per output word. All per-column and per-output addressing
+// is scalar: one buffer descriptor per launch, the column's byte offset as the
+// SGPR offset, the lane's word as the VGPR offset.
 #include <algorithm>
+#include <array>
+#include <cstdlib>
 #include <cstring>
+#include <unordered_map>
 
 #include "ihe_internal.h"
 
 namespace ihe {
 
-constexpr int kOmax = 12;        // outputs per group (register-blocked accumulators)
-constexpr int kPairChunk = 8;    // column pairs held in registers at a time
-constexpr int kColChunk = 8;     // columns per chunk in the wide kernel
-constexpr uint32_t kMaxCt = 1u << 22;
+constexpr int kOmax = 12;                    // outputs per group (register-blocked)
+constexpr int kOtab = 16;                    // output-table entries per group
+constexpr int kDot2ChunkCols = 16;           // 8 column pairs per chunk (dot2)
+constexpr int kWideChunkCols = 8;            // 8 columns per chunk (32-bit weights)
+constexpr uint32_t kSegCts = (1u << 19) - 1; // ciphertexts addressable by a 32-bit byte offset
+constexpr int kRsrcWord3 = 0x00020000;       // gfx9 raw buffer descriptor, dword 3
+
+#define IHE_CONSTANT __attribute__((address_space(4)))
+// Cache policy of the output stores (2 = nt, streaming).
+#ifndef IHE_STORE_POLICY
+#define IHE_STORE_POLICY 2
+#endif
+
+// Uniform metadata is read through the constant address space so the loads
+// become scalar (SMEM) loads.
+template <typename T>
+__device__ __forceinline__ const IHE_CONSTANT T *cptr(const T *p) {
+  return (const IHE_CONSTANT T *)p;
+}
 
 struct GroupDev {
-  uint32_t col_off;  // first packed column of the group
-  uint32_t nunits;   // column pairs (dot2) or columns (wide)
-  uint32_t out_off;  // first entry in outs[]
+  uint32_t col_off;  // first column word (multiple of 16)
+  uint32_t nchunks;  // chunks of columns
   uint32_t nout;     // outputs in the group (<= kOmax)
-  uint32_t w_off;    // first weight word of the [nout][nunits] block
+  uint32_t w_off;    // weights of chunk c, output o at w_off + (c * nout + o) * 8
+  uint32_t otab;     // first entry of the group's output tables (multiple of 16)
   uint32_t pad0, pad1, pad2;
 };
 
 struct CloudParams {
-  const GroupDev *__restrict__ grp;
-  const uint32_t *__restrict__ tile_off;
-  const uint32_t *__restrict__ cols;
-  const uint32_t *__restrict__ outs;
-  const int32_t *__restrict__ wts;
-  const uint32_t *__restrict__ bias;
-  const int32_t *__restrict__ in;
-  int32_t *__restrict__ out;
-  uint32_t ns, rs, nslices;
+  const GroupDev *grp;
+  const uint32_t *tile_off;
+  const uint32_t *cols;
+  const int32_t *wts;
+  const uint32_t *oslot;  // output byte offset (slot * 8192) relative to out
+  const uint32_t *obias;  // bias * 2^18
+  const int32_t *in;      // segment input base
+  int32_t *out;           // segment output base
+  uint32_t in_bytes, out_bytes;
+  uint32_t ns, rs, nslices, tile_base;
 };
 
 typedef short short2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
 
-// Word `word` (0..2047: a then b) of column cw = (ct << 10 | shift), rotated.
-template <bool ROT>
-__device__ __forceinline__ int32_t load_col(const int32_t *__restrict__ in, uint32_t cw,
-                                            uint32_t word) {
-  const int32_t *base = in + (size_t)(cw >> 10) * kCtWords;
-  if constexpr (!ROT) {
-    return base[word];
-  } else {
-    const uint32_t i = (word & (kN - 1)) + (cw & (kN - 1));
-    const int32_t x = base[(word & kN) + (i & (kN - 1))];
-    return i >= kN ? (int32_t)(0u - (uint32_t)x) : x;
-  }
+__device__ __forceinline__ GroupDev load_group(const GroupDev *grp, uint32_t g) {
+  return __builtin_bit_cast(GroupDev, *(const IHE_CONSTANT u32x8 *)(grp + g));
 }
 
 __device__ __forceinline__ int32_t dot2(uint32_t a, uint32_t b, int32_t c) {
@@ -78,151 +93,198 @@ __device__ __forceinline__ int32_t dot2(uint32_t a, uint32_t b, int32_t c) {
                                 false);
 }
 
-__device__ __forceinline__ void store_out(int32_t *p, uint32_t v) {
-  __builtin_nontemporal_store((int32_t)v, p);
-}
+// Per-lane addressing of the words a lane owns (word = 0..2047, a then b).
+template <int WPL>
+struct Lane {
+  uint32_t voff[WPL];    // word * 4
+  uint32_t wlo[WPL];     // word & (N-1)
+  uint32_t wpoly4[WPL];  // (word & N) * 4
+};
 
-template <int WPL, bool ROT>
-__global__ __launch_bounds__(256) void k_cloud_dot2(CloudParams P) {
-  const uint32_t slice = blockIdx.x % P.nslices;
-  const uint32_t tile = blockIdx.x / P.nslices;
-  const uint32_t wbase = slice * (256u * WPL);
-  const bool is_b = wbase >= kN;
-  uint32_t word[WPL];
+// Column word cw = byte offset of the ciphertext | shift << 2 (shift < N).
+// rho_shift(P)[i] = P[i + shift], negated past N.
+template <int WPL, bool ROT, int NC>
+__device__ __forceinline__ void load_chunk(__amdgpu_buffer_rsrc_t in_r,
+                                           const IHE_CONSTANT uint32_t *cols, uint32_t off,
+                                           const Lane<WPL> &L, uint32_t (&x)[NC][WPL]) {
+  uint32_t cw[NC];
+  if constexpr (NC == 16) {
+    const u32x16 v = *(const IHE_CONSTANT u32x16 *)(cols + off);
 #pragma unroll
-  for (int u = 0; u < WPL; ++u) word[u] = wbase + (uint32_t)u * 256u + threadIdx.x;
-
-  const uint32_t g_end = P.tile_off[tile + 1];
-  for (uint32_t g = P.tile_off[tile]; g < g_end; ++g) {
-    const GroupDev G = P.grp[g];
-    int32_t alo[kOmax][WPL], ahi[kOmax][WPL];
+    for (int k = 0; k < NC; ++k) cw[k] = v[k];
+  } else {
+    const u32x8 v = *(const IHE_CONSTANT u32x8 *)(cols + off);
 #pragma unroll
-    for (int o = 0; o < kOmax; ++o)
+    for (int k = 0; k < NC; ++k) cw[k] = v[k];
+  }
 #pragma unroll
-      for (int u = 0; u < WPL; ++u) alo[o][u] = ahi[o][u] = 0;
-
-    for (uint32_t p0 = 0; p0 < G.nunits; p0 += kPairChunk) {
-      uint32_t lo[kPairChunk][WPL], hi[kPairChunk][WPL];
+  for (int k = 0; k < NC; ++k) {
+    if constexpr (!ROT) {
 #pragma unroll
-      for (int pp = 0; pp < kPairChunk; ++pp) {
-        if (p0 + pp < G.nunits) {
-          const uint32_t c0 = P.cols[G.col_off + 2 * (p0 + pp)];
-          const uint32_t c1 = P.cols[G.col_off + 2 * (p0 + pp) + 1];
+      for (int u = 0; u < WPL; ++u)
+        x[k][u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(in_r, L.voff[u], (int)cw[k], 0);
+    } else {
+      const uint32_t soff = cw[k] & ~0x1FFFu;
+      const uint32_t sh = (cw[k] >> 2) & (kN - 1);
 #pragma unroll
-          for (int u = 0; u < WPL; ++u) {
-            const uint32_t x0 = (uint32_t)load_col<ROT>(P.in, c0, word[u]);
-            const uint32_t x1 = (uint32_t)load_col<ROT>(P.in, c1, word[u]);
-            lo[pp][u] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);
-            hi[pp][u] = __builtin_amdgcn_perm(x1 + 0x8000u, x0 + 0x8000u, 0x07060302u);
-          }
-        } else {
-#pragma unroll
-          for (int u = 0; u < WPL; ++u) lo[pp][u] = hi[pp][u] = 0;
-        }
-      }
-#pragma unroll
-      for (int o = 0; o < kOmax; ++o) {
-        if (o < (int)G.nout) {
-          const int32_t *wrow = P.wts + G.w_off + (uint32_t)o * G.nunits + p0;
-#pragma unroll
-          for (int pp = 0; pp < kPairChunk; ++pp) {
-            if (p0 + pp < G.nunits) {
-              const uint32_t w = (uint32_t)wrow[pp];
-#pragma unroll
-              for (int u = 0; u < WPL; ++u) {
-                alo[o][u] = dot2(lo[pp][u], w, alo[o][u]);
-                ahi[o][u] = dot2(hi[pp][u], w, ahi[o][u]);
-              }
-            }
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int o = 0; o < kOmax; ++o) {
-      if (o < (int)G.nout) {
-        const uint32_t slot = P.outs[G.out_off + o];
-        const uint32_t bias = is_b ? P.bias[slot] : 0u;
-        int32_t *dst = P.out + (size_t)slot * kCtWords;
-#pragma unroll
-        for (int u = 0; u < WPL; ++u) {
-          uint32_t v = (uint32_t)alo[o][u] + ((uint32_t)ahi[o][u] << 16);
-          if (is_b) {
-            const uint32_t i = word[u] - kN;
-            if (i < P.ns) v += bias;
-            if (i >= P.rs) v = 0;
-          }
-          store_out(dst + word[u], v);
-        }
+      for (int u = 0; u < WPL; ++u) {
+        const uint32_t t = L.wlo[u] + sh;
+        const uint32_t v = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
+            in_r, ((t & (kN - 1)) << 2) + L.wpoly4[u], (int)soff, 0);
+        x[k][u] = t >= kN ? 0u - v : v;
       }
     }
   }
 }
 
-// Weights outside int16: one 32-bit multiply per column and output word.
-template <int WPL, bool ROT>
-__global__ __launch_bounds__(256) void k_cloud_wide(CloudParams P) {
+template <int WPL, bool ROT, bool WIDE>
+__global__ __launch_bounds__(256) void k_cloud(CloudParams P) {
+  constexpr int NC = WIDE ? kWideChunkCols : kDot2ChunkCols;
+  constexpr int NP = NC / 2;  // column pairs per chunk (dot2)
   const uint32_t slice = blockIdx.x % P.nslices;
-  const uint32_t tile = blockIdx.x / P.nslices;
+  const uint32_t tile = P.tile_base + blockIdx.x / P.nslices;
   const uint32_t wbase = slice * (256u * WPL);
   const bool is_b = wbase >= kN;
-  uint32_t word[WPL];
+  const __amdgpu_buffer_rsrc_t in_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)P.in, (short)0, (int)P.in_bytes, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t out_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)P.out, (short)0, (int)P.out_bytes, kRsrcWord3);
+  Lane<WPL> L;
+  uint32_t bias_mask[WPL], keep_mask[WPL];
 #pragma unroll
-  for (int u = 0; u < WPL; ++u) word[u] = wbase + (uint32_t)u * 256u + threadIdx.x;
+  for (int u = 0; u < WPL; ++u) {
+    const uint32_t word = wbase + (uint32_t)u * 256u + threadIdx.x;
+    L.voff[u] = word * 4u;
+    L.wlo[u] = word & (kN - 1);
+    L.wpoly4[u] = (word & kN) * 4u;
+    // b polynomial: samples that get the bias, words that stay visible
+    const uint32_t i = word - kN;
+    bias_mask[u] = i < P.ns ? 0xFFFFFFFFu : 0u;
+    keep_mask[u] = i >= P.rs ? 0u : 0xFFFFFFFFu;
+  }
 
-  const uint32_t g_end = P.tile_off[tile + 1];
-  for (uint32_t g = P.tile_off[tile]; g < g_end; ++g) {
-    const GroupDev G = P.grp[g];
-    uint32_t acc[kOmax][WPL];
+  const IHE_CONSTANT uint32_t *cols = cptr(P.cols);
+  const IHE_CONSTANT uint32_t *wts = cptr((const uint32_t *)P.wts);
+  const IHE_CONSTANT uint32_t *toff = cptr(P.tile_off);
+
+  uint32_t g = toff[tile];
+  const uint32_t g_end = toff[tile + 1];
+  if (g >= g_end) return;
+
+  // The chunk being computed was loaded into xn one step earlier. A finished
+  // group's words stay in alo[] and are stored during the next iteration,
+  // before that iteration issues its loads: stores share the in-order vector
+  // memory counter with loads, so storing late keeps the wait for the next
+  // chunk from also waiting on fresh stores.
+  uint32_t xn[NC][WPL];
+  GroupDev G = load_group(P.grp, g);
+  uint32_t c = 0;
+  load_chunk<WPL, ROT, NC>(in_r, cols, G.col_off, L, xn);
+
+  uint32_t alo[kOmax][WPL], ahi[kOmax][WPL];
 #pragma unroll
-    for (int o = 0; o < kOmax; ++o)
+  for (int o = 0; o < kOmax; ++o)
 #pragma unroll
-      for (int u = 0; u < WPL; ++u) acc[o][u] = 0;
-    for (uint32_t c0 = 0; c0 < G.nunits; c0 += kColChunk) {
-      uint32_t x[kColChunk][WPL];
+    for (int u = 0; u < WPL; ++u) alo[o][u] = ahi[o][u] = 0;
+  uint32_t pend_nout = 0;
+  u32x16 pend_slots = {};
+
+  while (true) {
+    // operands of this chunk
+    uint32_t lo[WIDE ? NC : NP][WPL], hi[WIDE ? 1 : NP][WPL];
+    if constexpr (WIDE) {
 #pragma unroll
-      for (int cc = 0; cc < kColChunk; ++cc) {
-        if (c0 + cc < G.nunits) {
-          const uint32_t cw = P.cols[G.col_off + c0 + cc];
+      for (int k = 0; k < NC; ++k)
 #pragma unroll
-          for (int u = 0; u < WPL; ++u) x[cc][u] = (uint32_t)load_col<ROT>(P.in, cw, word[u]);
-        } else {
+        for (int u = 0; u < WPL; ++u) lo[k][u] = xn[k][u];
+    } else {
 #pragma unroll
-          for (int u = 0; u < WPL; ++u) x[cc][u] = 0;
+      for (int pp = 0; pp < NP; ++pp)
+#pragma unroll
+        for (int u = 0; u < WPL; ++u) {
+          const uint32_t x0 = xn[2 * pp][u], x1 = xn[2 * pp + 1][u];
+          lo[pp][u] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);
+          hi[pp][u] = __builtin_amdgcn_perm(x1 + 0x8000u, x0 + 0x8000u, 0x07060302u);
         }
-      }
+    }
+    // stores of the group finished last iteration
+    if (pend_nout) {
 #pragma unroll
       for (int o = 0; o < kOmax; ++o) {
-        if (o < (int)G.nout) {
-          const int32_t *wrow = P.wts + G.w_off + (uint32_t)o * G.nunits + c0;
+        if (o < (int)pend_nout) {
 #pragma unroll
-          for (int cc = 0; cc < kColChunk; ++cc) {
-            if (c0 + cc < G.nunits) {
-              const uint32_t w = (uint32_t)wrow[cc];
+          for (int u = 0; u < WPL; ++u) {
+            __builtin_amdgcn_raw_buffer_store_b32((int)alo[o][u], out_r, L.voff[u],
+                                                  (int)pend_slots[o], IHE_STORE_POLICY);
+            alo[o][u] = ahi[o][u] = 0;
+          }
+        }
+      }
+      pend_nout = 0;
+    }
+    // advance and start the next chunk's loads before computing this one
+    const bool last_chunk = c + 1 >= G.nchunks;
+    const GroupDev Gc = G;
+    const uint32_t cc = c;
+    bool more = true;
+    if (!last_chunk) {
+      ++c;
+    } else if (g + 1 < g_end) {
+      ++g;
+      G = load_group(P.grp, g);
+      c = 0;
+    } else {
+      more = false;
+    }
+    if (more) load_chunk<WPL, ROT, NC>(in_r, cols, G.col_off + (uint32_t)NC * c, L, xn);
+
+    // outputs of this chunk; the next output's weights load while this one computes
+    const IHE_CONSTANT uint32_t *wc = wts + Gc.w_off + cc * Gc.nout * 8u;
+    u32x8 wnext = *(const IHE_CONSTANT u32x8 *)wc;
 #pragma unroll
-              for (int u = 0; u < WPL; ++u) acc[o][u] += w * x[cc][u];
+    for (int o = 0; o < kOmax; ++o) {
+      if (o < (int)Gc.nout) {
+        const u32x8 w = wnext;
+        if (o + 1 < (int)Gc.nout) wnext = *(const IHE_CONSTANT u32x8 *)(wc + 8 * (o + 1));
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+          for (int u = 0; u < WPL; ++u) {
+            if constexpr (WIDE) {
+              alo[o][u] += w[k] * lo[k][u];
+            } else {
+              alo[o][u] = (uint32_t)dot2(lo[k][u], w[k], (int32_t)alo[o][u]);
+              ahi[o][u] = (uint32_t)dot2(hi[k][u], w[k], (int32_t)ahi[o][u]);
             }
+          }
+      }
+    }
+    if (last_chunk) {
+      const u32x16 bias = *(const IHE_CONSTANT u32x16 *)(cptr(P.obias) + Gc.otab);
+      pend_slots = *(const IHE_CONSTANT u32x16 *)(cptr(P.oslot) + Gc.otab);
+      pend_nout = Gc.nout;
+#pragma unroll
+      for (int o = 0; o < kOmax; ++o) {
+        if (o < (int)Gc.nout) {
+#pragma unroll
+          for (int u = 0; u < WPL; ++u) {
+            uint32_t v = WIDE ? alo[o][u] : alo[o][u] + (ahi[o][u] << 16);
+            if (is_b) v = (v + (bias[o] & bias_mask[u])) & keep_mask[u];
+            alo[o][u] = v;
           }
         }
       }
     }
+    if (!more) break;
+  }
+  // the last group's stores
 #pragma unroll
-    for (int o = 0; o < kOmax; ++o) {
-      if (o < (int)G.nout) {
-        const uint32_t slot = P.outs[G.out_off + o];
-        const uint32_t bias = is_b ? P.bias[slot] : 0u;
-        int32_t *dst = P.out + (size_t)slot * kCtWords;
+  for (int o = 0; o < kOmax; ++o) {
+    if (o < (int)pend_nout) {
 #pragma unroll
-        for (int u = 0; u < WPL; ++u) {
-          uint32_t v = acc[o][u];
-          if (is_b) {
-            const uint32_t i = word[u] - kN;
-            if (i < P.ns) v += bias;
-            if (i >= P.rs) v = 0;
-          }
-          store_out(dst + word[u], v);
-        }
-      }
+      for (int u = 0; u < WPL; ++u)
+        __builtin_amdgcn_raw_buffer_store_b32((int)alo[o][u], out_r, L.voff[u], (int)pend_slots[o],
+                                              IHE_STORE_POLICY);
     }
   }
 }
@@ -243,83 +305,231 @@ __global__ __launch_bounds__(256) void k_cloud_var(const uint32_t *__restrict__ 
   var_out[o] = v;
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged kernel for narrow windows (<= 32 columns per group, int16
+// weights). A workgroup owns one tile (consecutive groups) and one 256-word
+// slice. It first stages the tile's sliding window of input columns in LDS,
+// each column once (consecutive targets share most of their window), and
+// copies the tile's metadata (group headers, output slots, biases, weights)
+// into LDS. The compute phase then reads only LDS: column words are per-lane
+// LDS reads, weights are broadcast LDS reads that complete in order, so the
+// next output's weights are fetched while the current one computes, and the
+// output stores never queue behind vector loads.
+constexpr uint32_t kStageSlice = 256;    // words per workgroup slice
+constexpr uint32_t kStageMaxCols = 48;   // staged columns per tile
+constexpr uint32_t kStageMaxMeta = 2048; // metadata words per tile
+
+struct TileDev {
+  uint32_t stage_off;  // first column word in stage[] (multiple of 16)
+  uint32_t nstage;     // columns staged into LDS
+  uint32_t meta_off;   // first metadata word in meta[]
+  uint32_t nmeta;      // metadata words
+  uint32_t ngrp;       // groups
+  uint32_t pad0, pad1, pad2;
+};
+
+// Metadata blob of a tile (32-bit words):
+//   header of group i at 8*i: {first staged column, chunks, outputs,
+//                              weights offset, slot/bias offset}
+//   slots[nout] then biases[nout] at the slot/bias offset
+//   weights [chunk][output][8 packed int16 pairs] at the weights offset
+struct StagedParams {
+  const TileDev *tiles;
+  const uint32_t *stage;
+  const uint32_t *meta;
+  const int32_t *in;
+  int32_t *out;
+  uint32_t in_bytes, out_bytes;
+  uint32_t ns, rs, tile_base, meta_lds;  // meta_lds: word offset of the metadata in LDS
+};
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <bool ROT>
+__device__ __forceinline__ uint32_t load_word(__amdgpu_buffer_rsrc_t in_r, uint32_t cw,
+                                              uint32_t voff, uint32_t wlo, uint32_t wpoly4) {
+  if constexpr (!ROT) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(in_r, voff, (int)cw, 0);
+  } else {
+    const uint32_t t = wlo + ((cw >> 2) & (kN - 1));
+    const uint32_t v = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
+        in_r, ((t & (kN - 1)) << 2) + wpoly4, (int)(cw & ~0x1FFFu), 0);
+    return t >= kN ? 0u - v : v;
+  }
+}
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <bool ROT>
+__global__ __launch_bounds__(256) void k_cloud_staged(StagedParams P) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t slice = blockIdx.x & 7u;
+  const uint32_t tile = P.tile_base + (blockIdx.x >> 3);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t word = slice * kStageSlice + tid;
+  const bool is_b = slice >= 4;
+  const __amdgpu_buffer_rsrc_t in_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)P.in, (short)0, (int)P.in_bytes, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t out_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)P.out, (short)0, (int)P.out_bytes, kRsrcWord3);
+  const TileDev T = __builtin_bit_cast(TileDev, *(const IHE_CONSTANT u32x8 *)(P.tiles + tile));
+  const uint32_t voff = word * 4u, wlo = word & (kN - 1), wpoly4 = (word & kN) * 4u;
+  const uint32_t i_b = word - kN;
+  const uint32_t bias_mask = i_b < P.ns ? 0xFFFFFFFFu : 0u;
+  const uint32_t keep_mask = i_b >= P.rs ? 0u : 0xFFFFFFFFu;
+  uint32_t *meta = lds + P.meta_lds;
+
+  // phase 1: stage the tile's columns (all loads in flight together) and metadata
+  {
+    uint32_t x[kStageMaxCols / 16][16];
+#pragma unroll
+    for (int b = 0; b < (int)(kStageMaxCols / 16); ++b) {
+      if ((uint32_t)b * 16u < T.nstage) {
+        const u32x16 cw = *(const IHE_CONSTANT u32x16 *)(cptr(P.stage) + T.stage_off + 16 * b);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[b][k] = load_word<ROT>(in_r, cw[k], voff, wlo, wpoly4);
+      }
+    }
+    for (uint32_t i = tid; i < T.nmeta; i += kStageSlice) meta[i] = P.meta[T.meta_off + i];
+#pragma unroll
+    for (int b = 0; b < (int)(kStageMaxCols / 16); ++b) {
+      if ((uint32_t)b * 16u < T.nstage) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+          if ((uint32_t)(b * 16 + k) < T.nstage) lds[(b * 16 + k) * kStageSlice + tid] = x[b][k];
+      }
+    }
+  }
+  __syncthreads();
+
+  // phase 2: groups of the tile, from LDS only
+  for (uint32_t gi = 0; gi < T.ngrp; ++gi) {
+    const uint32_t s0 = rfl(meta[8 * gi]), nchunks = rfl(meta[8 * gi + 1]);
+    const uint32_t nout = rfl(meta[8 * gi + 2]), wo = rfl(meta[8 * gi + 3]);
+    const uint32_t oo = rfl(meta[8 * gi + 4]);
+    uint32_t alo[kOmax], ahi[kOmax];
+#pragma unroll
+    for (int o = 0; o < kOmax; ++o) alo[o] = ahi[o] = 0;
+    for (uint32_t c = 0; c < nchunks; ++c) {
+      const uint32_t *col = lds + (s0 + 16 * c) * kStageSlice + tid;
+      uint32_t lo[8], hi[8];
+#pragma unroll
+      for (int pp = 0; pp < 8; ++pp) {
+        const uint32_t x0 = col[(2 * pp) * kStageSlice], x1 = col[(2 * pp + 1) * kStageSlice];
+        lo[pp] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);
+        hi[pp] = __builtin_amdgcn_perm(x1 + 0x8000u, x0 + 0x8000u, 0x07060302u);
+      }
+      const uint32_t *wc = meta + wo + c * nout * 8u;
+#pragma unroll
+      for (int o = 0; o < kOmax; ++o) {
+        if (o < (int)nout) {
+          const u32x4 w0 = *(const u32x4 *)(wc + 8 * o);
+          const u32x4 w1 = *(const u32x4 *)(wc + 8 * o + 4);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            alo[o] = (uint32_t)dot2(lo[k], w0[k], (int32_t)alo[o]);
+            ahi[o] = (uint32_t)dot2(hi[k], w0[k], (int32_t)ahi[o]);
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            alo[o] = (uint32_t)dot2(lo[4 + k], w1[k], (int32_t)alo[o]);
+            ahi[o] = (uint32_t)dot2(hi[4 + k], w1[k], (int32_t)ahi[o]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < kOmax; ++o) {
+      if (o < (int)nout) {
+        const uint32_t slot = rfl(meta[oo + o]);
+        const uint32_t bias = rfl(meta[oo + nout + o]);
+        uint32_t v = alo[o] + (ahi[o] << 16);
+        if (is_b) v = (v + (bias & bias_mask)) & keep_mask;
+        __builtin_amdgcn_raw_buffer_store_b32((int)v, out_r, voff, (int)slot, IHE_STORE_POLICY);
+      }
+    }
+  }
+}
+
 }  // namespace ihe
 
 using namespace ihe;
+
+namespace {
+
+struct Segment {
+  uint32_t tile_begin, tile_end;  // tiles [begin, end)
+  uint32_t in_base, out_base;     // first input ciphertext / output slot addressed
+};
+
+}  // namespace
 
 struct ihe_band {
   ihe_ctx *ctx = nullptr;
   ihe_layout layout{};
   uint32_t num_in = 0, num_out = 0;
   int variant = IHE_VARIANT_VALU;
+  bool staged = false;
   int wpl = 1;
   uint32_t num_groups = 0, num_tiles = 0, num_cols = 0, nnz = 0, max_window = 0;
+  uint32_t lds_bytes = 0, meta_lds = 0;
+  std::vector<Segment> segments;
   void *dmem = nullptr;
   size_t dbytes = 0;
+  // direct layout
   GroupDev *d_grp = nullptr;
-  uint32_t *d_tile_off = nullptr, *d_cols = nullptr, *d_outs = nullptr, *d_bias = nullptr;
+  uint32_t *d_tile_off = nullptr, *d_cols = nullptr, *d_oslot = nullptr, *d_obias = nullptr;
   int32_t *d_wts = nullptr;
+  // staged layout
+  TileDev *d_tiles = nullptr;
+  uint32_t *d_stage = nullptr, *d_meta = nullptr;
+  // variances
   uint32_t *d_var_ptr = nullptr, *d_var_ct = nullptr;
   int32_t *d_var_w2 = nullptr;
 };
 
 namespace {
 
+struct HostGroup {
+  uint32_t ncol;                 // real columns
+  uint32_t col_begin;            // into HostBand::gcol (absolute big indices)
+  uint32_t out_begin, nout;      // into HostBand::gout
+  uint32_t w_begin;              // into HostBand::gw (nout rows of ncol weights)
+  uint32_t ct_min, ct_max;       // FeatIndex range of the real columns
+};
+
 struct HostBand {
+  bool wide = false, staged = false;
+  // direct layout
   std::vector<GroupDev> grp;
-  std::vector<uint32_t> tile_off, cols, outs, bias, var_ptr, var_ct;
-  std::vector<int32_t> wts, var_w2;
+  std::vector<uint32_t> tile_off, cols, oslot, obias;
+  std::vector<int32_t> wts;
+  // staged layout
+  std::vector<TileDev> tiles;
+  std::vector<uint32_t> stage, meta;
+  uint32_t max_stage = 0, max_meta = 0;
+  // both
+  std::vector<Segment> segments;
+  std::vector<uint32_t> var_ptr, var_ct, bias;
+  std::vector<int32_t> var_w2;
   uint32_t max_window = 0;
+  // grouping
+  std::vector<HostGroup> hg;
+  std::vector<uint32_t> gcol, gout;
+  std::vector<int32_t> gw;
 };
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int compile_band(const ihe_layout &L, uint32_t num_in, const ihe_model_csr &M, bool wide,
-                 uint32_t tile_work, HostBand &H) {
-  const uint32_t R = L.num_regions, RS = L.region_size;
+// Outputs (in slot order) with identical sorted column lists form groups of
+// up to kOmax outputs. Validates the model like the reference's lookups do.
+int build_groups(const ihe_layout &L, uint32_t num_in, const ihe_model_csr &M, bool wide,
+                 HostBand &H) {
+  const uint32_t R = L.num_regions;
   const uint32_t num_out = M.num_outputs;
   H.bias.assign(num_out, 0);
   H.var_ptr.assign(num_out + 1, 0);
-
-  std::vector<std::pair<uint32_t, int32_t>> row;      // sorted (bidx, w) of the current output
-  std::vector<uint32_t> gcols;                        // column list of the open group
-  std::vector<std::vector<int32_t>> gw;               // weight rows of the open group
-  std::vector<uint32_t> gouts;
-  const uint32_t unit = wide ? 1 : 2;
-
-  auto flush = [&]() {
-    if (gouts.empty()) return;
-    GroupDev G{};
-    const uint32_t ncol = (uint32_t)gcols.size();
-    const uint32_t nunits = (ncol + unit - 1) / unit;
-    G.col_off = (uint32_t)H.cols.size();
-    G.nunits = nunits;
-    G.out_off = (uint32_t)H.outs.size();
-    G.nout = (uint32_t)gouts.size();
-    G.w_off = (uint32_t)H.wts.size();
-    for (uint32_t k = 0; k < nunits * unit; ++k) {
-      const uint32_t c = gcols[std::min(k, ncol - 1)];
-      H.cols.push_back(((c / R) << 10) | ((c % R) * RS));
-    }
-    for (uint32_t o = 0; o < G.nout; ++o) {
-      const auto &w = gw[o];
-      for (uint32_t p = 0; p < nunits; ++p) {
-        if (wide) {
-          H.wts.push_back(w[p]);
-        } else {
-          const uint32_t lo = (uint16_t)(int16_t)w[2 * p];
-          const uint32_t hi = (2 * p + 1 < ncol) ? (uint16_t)(int16_t)w[2 * p + 1] : 0u;
-          H.wts.push_back((int32_t)(lo | (hi << 16)));
-        }
-      }
-    }
-    H.outs.insert(H.outs.end(), gouts.begin(), gouts.end());
-    H.max_window = std::max(H.max_window, ncol);
-    H.grp.push_back(G);
-    gouts.clear();
-    gw.clear();
-  };
-
+  std::vector<std::pair<uint32_t, int32_t>> row;
   for (uint32_t o = 0; o < num_out; ++o) {
     row.clear();
     bool has_const = false;
@@ -329,13 +539,12 @@ int compile_band(const ihe_layout &L, uint32_t num_in, const ihe_model_csr &M, b
       if (c == IHE_CONSTANT_BIDX) {
         if (has_const) return fail(IHE_ERR_ARG, "duplicate constant entry in output " + std::to_string(o));
         has_const = true;
-        H.bias[o] = (uint32_t)w * kOne;
+        H.bias[o] = (uint32_t)w * kOne;  // coeff * ONE_IN_T32, an int32 product that may wrap
         continue;
       }
       if (c / R >= num_in)
         return fail(IHE_ERR_RANGE, "output " + std::to_string(o) + " references FeatIndex " +
                                        std::to_string(c / R) + " >= " + std::to_string(num_in));
-      if (c / R >= kMaxCt) return fail(IHE_ERR_UNSUPPORTED, "FeatIndex beyond 2^22");
       if (!wide && (w < -32768 || w > 32767))
         return fail(IHE_ERR_UNSUPPORTED, "weight outside int16 for the dot2 variant");
       row.emplace_back(c, w);
@@ -349,39 +558,261 @@ int compile_band(const ihe_layout &L, uint32_t num_in, const ihe_model_csr &M, b
     for (size_t k = 1; k < row.size(); ++k)
       if (row[k].first == row[k - 1].first)
         return fail(IHE_ERR_ARG, "duplicate input index in output " + std::to_string(o));
-    bool same = !gouts.empty() && gouts.size() < (size_t)kOmax && gcols.size() == row.size();
-    for (size_t k = 0; same && k < row.size(); ++k) same = gcols[k] == row[k].first;
+    bool same = !H.hg.empty() && H.hg.back().nout < (uint32_t)kOmax && H.hg.back().ncol == row.size();
+    if (same) {
+      const uint32_t *gc = H.gcol.data() + H.hg.back().col_begin;
+      for (size_t k = 0; same && k < row.size(); ++k) same = gc[k] == row[k].first;
+    }
     if (!same) {
-      flush();
-      gcols.clear();
-      for (auto &p : row) gcols.push_back(p.first);
+      HostGroup hg{};
+      hg.ncol = (uint32_t)row.size();
+      hg.col_begin = (uint32_t)H.gcol.size();
+      hg.out_begin = (uint32_t)H.gout.size();
+      hg.w_begin = (uint32_t)H.gw.size();
+      hg.ct_min = row.empty() ? UINT32_MAX : row.front().first / R;
+      hg.ct_max = row.empty() ? 0 : row.back().first / R;
+      for (auto &p : row) H.gcol.push_back(p.first);
+      H.hg.push_back(hg);
+      if (hg.ncol > 0 && hg.ct_max - hg.ct_min >= kSegCts)
+        return fail(IHE_ERR_UNSUPPORTED, "an output spans more than 4 GiB of input ciphertexts");
     }
-    gouts.push_back(o);
-    std::vector<int32_t> w(row.size());
-    for (size_t k = 0; k < row.size(); ++k) w[k] = row[k].second;
-    gw.push_back(std::move(w));
+    HostGroup &hg = H.hg.back();
+    H.gout.push_back(o);
+    hg.nout++;
+    for (auto &p : row) H.gw.push_back(p.second);
+    H.max_window = std::max(H.max_window, (uint32_t)row.size());
   }
-  flush();
-
-  // Tiles of roughly tile_work (outputs x units) each.
-  H.tile_off.push_back(0);
-  uint64_t acc = 0;
-  for (uint32_t g = 0; g < H.grp.size(); ++g) {
-    acc += (uint64_t)H.grp[g].nout * (H.grp[g].nunits + 1);
-    if (acc >= tile_work) {
-      H.tile_off.push_back(g + 1);
-      acc = 0;
-    }
-  }
-  if (H.tile_off.back() != H.grp.size()) H.tile_off.push_back((uint32_t)H.grp.size());
   return IHE_OK;
 }
 
-template <typename T>
-T *carve(char *base, size_t &off, size_t count) {
-  T *p = reinterpret_cast<T *>(base + off);
-  off += align_up(sizeof(T) * std::max<size_t>(count, 1));
-  return p;
+// Packs column big index c as (byte offset of its ciphertext relative to the
+// segment's first one) | (rotation shift << 2).
+uint32_t col_word(uint32_t c, uint32_t R, uint32_t RS, uint32_t in_base) {
+  return (((c / R) - in_base) << 13) | (((c % R) * RS) << 2);
+}
+
+// Grows segments over consecutive units (tiles) while their input and output
+// ranges stay addressable with 32-bit byte offsets.
+struct SegmentBuilder {
+  std::vector<Segment> &out;
+  uint32_t first = 0, in_lo = UINT32_MAX, in_hi = 0, out_lo = UINT32_MAX, out_hi = 0;
+  // returns the segment bases the unit was placed in
+  void add(uint32_t unit, uint32_t ilo, uint32_t ihi, uint32_t olo, uint32_t ohi) {
+    const uint32_t nil = std::min(in_lo, ilo), nih = std::max(in_hi, ihi);
+    const uint32_t nol = std::min(out_lo, olo), noh = std::max(out_hi, ohi);
+    const bool in_ok = nil == UINT32_MAX || nih - nil < kSegCts;
+    if (unit > first && (!in_ok || noh - nol >= kSegCts)) {
+      close(unit);
+      in_lo = ilo, in_hi = ihi, out_lo = olo, out_hi = ohi;
+    } else {
+      in_lo = nil, in_hi = nih, out_lo = nol, out_hi = noh;
+    }
+  }
+  void close(uint32_t unit_end) {
+    if (unit_end > first) out.push_back({first, unit_end, in_lo == UINT32_MAX ? 0 : in_lo, out_lo});
+    first = unit_end;
+    in_lo = out_lo = UINT32_MAX;
+    in_hi = out_hi = 0;
+  }
+};
+
+// Direct layout: tiles of roughly tile_work units, columns read from HBM/L2 per group.
+void layout_direct(const ihe_layout &L, uint32_t tile_work, HostBand &H) {
+  const uint32_t R = L.num_regions, RS = L.region_size;
+  const uint32_t chunk_cols = H.wide ? kWideChunkCols : kDot2ChunkCols;
+  H.tile_off.assign(1, 0);
+  std::vector<std::array<uint32_t, 4>> tile_rng;  // in lo/hi, out lo/hi per tile
+  uint64_t acc = 0;
+  std::array<uint32_t, 4> rng = {UINT32_MAX, 0, UINT32_MAX, 0};
+  for (uint32_t g = 0; g < H.hg.size(); ++g) {
+    const HostGroup &hg = H.hg[g];
+    rng[0] = std::min(rng[0], hg.ct_min);
+    rng[1] = std::max(rng[1], hg.ct_max);
+    rng[2] = std::min(rng[2], H.gout[hg.out_begin]);
+    rng[3] = std::max(rng[3], H.gout[hg.out_begin + hg.nout - 1]);
+    const uint32_t nchunks = std::max<uint32_t>(1, (hg.ncol + chunk_cols - 1) / chunk_cols);
+    acc += (uint64_t)hg.nout * (nchunks * 8 + 1);
+    // a tile must fit one segment; a large spread starts a new tile
+    const bool spread = rng[1] != 0 && rng[0] != UINT32_MAX && rng[1] - rng[0] >= kSegCts / 2;
+    if (acc >= tile_work || g + 1 == H.hg.size() || spread) {
+      H.tile_off.push_back(g + 1);
+      tile_rng.push_back(rng);
+      acc = 0;
+      rng = {UINT32_MAX, 0, UINT32_MAX, 0};
+    }
+  }
+  SegmentBuilder sb{H.segments};
+  for (uint32_t t = 0; t + 1 < H.tile_off.size(); ++t)
+    sb.add(t, tile_rng[t][0], tile_rng[t][1], tile_rng[t][2], tile_rng[t][3]);
+  sb.close((uint32_t)H.tile_off.size() - 1);
+
+  for (const Segment &s : H.segments) {
+    for (uint32_t t = s.tile_begin; t < s.tile_end; ++t) {
+      for (uint32_t g = H.tile_off[t]; g < H.tile_off[t + 1]; ++g) {
+        const HostGroup &hg = H.hg[g];
+        GroupDev G{};
+        const uint32_t ncol_pad = std::max<uint32_t>(16, (hg.ncol + 15) / 16 * 16);
+        G.col_off = (uint32_t)H.cols.size();
+        G.nchunks = ncol_pad / chunk_cols;
+        G.nout = hg.nout;
+        G.w_off = (uint32_t)H.wts.size();
+        G.otab = (uint32_t)H.oslot.size();
+        for (uint32_t k = 0; k < ncol_pad; ++k) {
+          // padding repeats the last real column (weight 0) or reads the
+          // segment's first ciphertext
+          const uint32_t c =
+              hg.ncol ? H.gcol[hg.col_begin + std::min(k, hg.ncol - 1)] : s.in_base * R;
+          H.cols.push_back(col_word(c, R, RS, s.in_base));
+        }
+        for (uint32_t ch = 0; ch < G.nchunks; ++ch) {
+          for (uint32_t o = 0; o < hg.nout; ++o) {
+            const int32_t *w = H.gw.data() + hg.w_begin + (size_t)o * hg.ncol;
+            for (uint32_t k = 0; k < 8; ++k) {
+              if (H.wide) {
+                const uint32_t col = ch * 8 + k;
+                H.wts.push_back(col < hg.ncol ? w[col] : 0);
+              } else {
+                const uint32_t c0 = ch * 16 + 2 * k, c1 = c0 + 1;
+                const uint32_t lo = c0 < hg.ncol ? (uint16_t)(int16_t)w[c0] : 0u;
+                const uint32_t hi = c1 < hg.ncol ? (uint16_t)(int16_t)w[c1] : 0u;
+                H.wts.push_back((int32_t)(lo | (hi << 16)));
+              }
+            }
+          }
+        }
+        for (uint32_t o = 0; o < (uint32_t)kOtab; ++o) {
+          const uint32_t slot = o < hg.nout ? H.gout[hg.out_begin + o] : s.out_base;
+          H.oslot.push_back((slot - s.out_base) * kCtWords * 4u);
+          H.obias.push_back(o < hg.nout ? H.bias[slot] : 0u);
+        }
+        H.grp.push_back(G);
+      }
+    }
+  }
+}
+
+// Staged layout: tiles whose staged column list (the union of their groups'
+// windows, shared where consecutive windows overlap) fits the LDS budget.
+void layout_staged(const ihe_layout &L, uint32_t tile_outputs, HostBand &H) {
+  const uint32_t R = L.num_regions, RS = L.region_size;
+  struct TileHost {
+    std::vector<uint32_t> S;                       // staged column big indices
+    std::vector<std::pair<uint32_t, uint32_t>> gs; // (group, first staged column)
+    uint32_t nstage_pad = 0, nmeta = 0, nout = 0;
+    uint32_t in_lo = UINT32_MAX, in_hi = 0, out_lo = UINT32_MAX, out_hi = 0;
+  };
+  std::vector<TileHost> tiles;
+  TileHost cur;
+  std::unordered_map<uint32_t, uint32_t> pos;  // big index -> position in cur.S
+  auto meta_of = [](const HostGroup &hg, uint32_t nchunks) {
+    return 8u + (2 * hg.nout + 3) / 4 * 4 + nchunks * hg.nout * 8u;
+  };
+  // Where a group's sorted columns go in the staged list: inside it, or
+  // overlapping its tail (a sliding window), or appended whole. Returns the
+  // first staged column and how many columns would be appended.
+  auto probe = [&](const HostGroup &hg, const TileHost &T) -> std::pair<uint32_t, uint32_t> {
+    if (hg.ncol == 0) return {0u, 0u};
+    const uint32_t *C = H.gcol.data() + hg.col_begin;
+    auto it = pos.find(C[0]);
+    if (it != pos.end()) {
+      const uint32_t p0 = it->second;
+      uint32_t m = 0;
+      while (m < hg.ncol && p0 + m < T.S.size() && T.S[p0 + m] == C[m]) ++m;
+      if (m == hg.ncol || p0 + m == T.S.size()) return {p0, hg.ncol - m};
+    }
+    return {(uint32_t)T.S.size(), hg.ncol};
+  };
+  auto fits = [&](const HostGroup &hg, const TileHost &T) {
+    const uint32_t nchunks = std::max<uint32_t>(1, (hg.ncol + 15) / 16);
+    const auto [s0, added] = probe(hg, T);
+    const uint32_t nstage = std::max({T.nstage_pad, (uint32_t)T.S.size() + added, s0 + 16 * nchunks});
+    return nstage <= kStageMaxCols && T.nmeta + meta_of(hg, nchunks) <= kStageMaxMeta &&
+           T.nout + hg.nout <= tile_outputs;
+  };
+  auto place = [&](const HostGroup &hg, TileHost &T) {
+    const auto [s0, added] = probe(hg, T);
+    const uint32_t *C = H.gcol.data() + hg.col_begin;
+    for (uint32_t k = hg.ncol - added; k < hg.ncol; ++k) {
+      pos[C[k]] = (uint32_t)T.S.size();
+      T.S.push_back(C[k]);
+    }
+    return s0;
+  };
+  for (uint32_t g = 0; g < H.hg.size(); ++g) {
+    const HostGroup &hg = H.hg[g];
+    if (!cur.gs.empty() && !fits(hg, cur)) {
+      tiles.push_back(std::move(cur));
+      cur = TileHost();
+      pos.clear();
+    }
+    const uint32_t nchunks = std::max<uint32_t>(1, (hg.ncol + 15) / 16);
+    const uint32_t s0 = place(hg, cur);
+    cur.gs.emplace_back(g, s0);
+    cur.nstage_pad = std::max(cur.nstage_pad, std::max<uint32_t>((uint32_t)cur.S.size(), s0 + 16 * nchunks));
+    cur.nmeta += meta_of(hg, nchunks);
+    cur.nout += hg.nout;
+    cur.in_lo = std::min(cur.in_lo, hg.ct_min);
+    cur.in_hi = std::max(cur.in_hi, hg.ct_max);
+    cur.out_lo = std::min(cur.out_lo, H.gout[hg.out_begin]);
+    cur.out_hi = std::max(cur.out_hi, H.gout[hg.out_begin + hg.nout - 1]);
+  }
+  if (!cur.gs.empty()) tiles.push_back(std::move(cur));
+
+  SegmentBuilder sb{H.segments};
+  for (uint32_t t = 0; t < tiles.size(); ++t)
+    sb.add(t, tiles[t].in_lo, tiles[t].in_hi, tiles[t].out_lo, tiles[t].out_hi);
+  sb.close((uint32_t)tiles.size());
+
+  for (const Segment &seg : H.segments) {
+    for (uint32_t t = seg.tile_begin; t < seg.tile_end; ++t) {
+      TileHost &T = tiles[t];
+      TileDev D{};
+      // staged columns, padded by repeats of the last one to whole chunks and
+      // to a multiple of 16 entries
+      const uint32_t nstage = std::max<uint32_t>(T.nstage_pad, 16);
+      D.stage_off = (uint32_t)H.stage.size();
+      D.nstage = nstage;
+      const uint32_t fill = T.S.empty() ? seg.in_base * R : T.S.back();
+      for (uint32_t k = 0; k < (nstage + 15) / 16 * 16; ++k)
+        H.stage.push_back(col_word(k < T.S.size() ? T.S[k] : fill, R, RS, seg.in_base));
+      // metadata blob
+      D.meta_off = (uint32_t)H.meta.size();
+      D.ngrp = (uint32_t)T.gs.size();
+      std::vector<uint32_t> blob(8 * T.gs.size(), 0);
+      for (uint32_t i = 0; i < T.gs.size(); ++i) {
+        const HostGroup &hg = H.hg[T.gs[i].first];
+        blob[8 * i] = T.gs[i].second;
+        blob[8 * i + 1] = std::max<uint32_t>(1, (hg.ncol + 15) / 16);
+        blob[8 * i + 2] = hg.nout;
+        blob[8 * i + 4] = (uint32_t)blob.size();
+        for (uint32_t o = 0; o < hg.nout; ++o)
+          blob.push_back((H.gout[hg.out_begin + o] - seg.out_base) * kCtWords * 4u);
+        for (uint32_t o = 0; o < hg.nout; ++o) blob.push_back(H.bias[H.gout[hg.out_begin + o]]);
+        while (blob.size() % 4) blob.push_back(0);
+      }
+      for (uint32_t i = 0; i < T.gs.size(); ++i) {
+        const HostGroup &hg = H.hg[T.gs[i].first];
+        const uint32_t nchunks = blob[8 * i + 1];
+        blob[8 * i + 3] = (uint32_t)blob.size();
+        for (uint32_t ch = 0; ch < nchunks; ++ch)
+          for (uint32_t o = 0; o < hg.nout; ++o) {
+            const int32_t *w = H.gw.data() + hg.w_begin + (size_t)o * hg.ncol;
+            for (uint32_t k = 0; k < 8; ++k) {
+              const uint32_t c0 = ch * 16 + 2 * k, c1 = c0 + 1;
+              const uint32_t lo = c0 < hg.ncol ? (uint16_t)(int16_t)w[c0] : 0u;
+              const uint32_t hi = c1 < hg.ncol ? (uint16_t)(int16_t)w[c1] : 0u;
+              blob.push_back(lo | (hi << 16));
+            }
+          }
+      }
+      D.nmeta = (uint32_t)blob.size();
+      H.meta.insert(H.meta.end(), blob.begin(), blob.end());
+      H.max_stage = std::max(H.max_stage, nstage);
+      H.max_meta = std::max(H.max_meta, D.nmeta);
+      H.tiles.push_back(D);
+      H.cols.resize(H.cols.size() + T.S.size());  // counts staged columns for the band info
+    }
+  }
 }
 
 int check_layout(const ihe_layout *L) {
@@ -392,13 +823,17 @@ int check_layout(const ihe_layout *L) {
   return IHE_OK;
 }
 
-}  // namespace
-
-namespace {
+uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi) {
+  if (const char *e = std::getenv(name)) {
+    const long v = std::atol(e);
+    if (v >= (long)lo && v <= (long)hi) return (uint32_t)v;
+  }
+  return dflt;
+}
 
 // Validates arguments, picks the kernel family and compiles the band on the host.
 int plan_band(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr *model,
-              int variant, HostBand &H, bool &wide) {
+              int variant, HostBand &H) {
   if (!model) return fail(IHE_ERR_ARG, "null model");
   if (int rc = check_layout(layout)) return rc;
   if (model->num_outputs > 0 && (!model->row_ptr || (model->row_ptr[model->num_outputs] > 0 &&
@@ -407,7 +842,7 @@ int plan_band(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr
   if (variant != IHE_VARIANT_AUTO && variant != IHE_VARIANT_VALU && variant != IHE_VARIANT_WIDE &&
       variant != IHE_VARIANT_MFMA)
     return fail(IHE_ERR_ARG, "unknown variant");
-  wide = variant == IHE_VARIANT_WIDE;
+  bool wide = variant == IHE_VARIANT_WIDE;
   if (variant == IHE_VARIANT_AUTO || variant == IHE_VARIANT_MFMA) {
     // Weights beyond int16 need the 32-bit multiply kernel.
     const uint32_t nnz = model->num_outputs ? model->row_ptr[model->num_outputs] : 0;
@@ -421,18 +856,32 @@ int plan_band(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr
       return fail(IHE_ERR_UNSUPPORTED, "MFMA variant needs weights within int16");
   }
   if (variant == IHE_VARIANT_MFMA) return fail(IHE_ERR_UNSUPPORTED, "MFMA variant not built");
-  return compile_band(*layout, num_in_cts, *model, wide, 512, H);
+  H.wide = wide;
+  if (int rc = build_groups(*layout, num_in_cts, *model, wide, H)) return rc;
+  // Narrow windows with int16 weights use the LDS-staged kernel.
+  const uint32_t stage_mode = env_u32("IHE_CLOUD_STAGED", 1, 0, 1);
+  H.staged = stage_mode && !wide && H.max_window <= kStageMaxCols - 16;
+  if (H.staged) layout_staged(*layout, env_u32("IHE_CLOUD_TILE_OUTPUTS", 48, 1, 4096), H);
+  else layout_direct(*layout, env_u32("IHE_CLOUD_TILE_WORK", 512, 8, 1 << 20), H);
+  return IHE_OK;
 }
 
-void fill_info(const HostBand &H, const ihe_model_csr *model, bool wide, ihe_band_info *info) {
+void fill_info(const HostBand &H, const ihe_model_csr *model, ihe_band_info *info) {
   info->num_outputs = model->num_outputs;
-  info->num_groups = (uint32_t)H.grp.size();
-  info->num_tiles = (uint32_t)H.tile_off.size() - 1;
+  info->num_groups = (uint32_t)H.hg.size();
+  info->num_tiles = H.staged ? (uint32_t)H.tiles.size() : (uint32_t)H.tile_off.size() - 1;
   info->num_cols = (uint32_t)H.cols.size();
   info->nnz = model->num_outputs ? model->row_ptr[model->num_outputs] : 0;
-  info->variant = wide ? IHE_VARIANT_WIDE : IHE_VARIANT_VALU;
+  info->variant = H.wide ? IHE_VARIANT_WIDE : IHE_VARIANT_VALU;
   info->max_window = H.max_window;
   info->device_bytes = 0;
+}
+
+template <typename T>
+T *carve(char *base, size_t &off, size_t count) {
+  T *p = reinterpret_cast<T *>(base + off);
+  off += align_up(sizeof(T) * std::max<size_t>(count, 1));
+  return p;
 }
 
 }  // namespace
@@ -443,9 +892,8 @@ int ihe_band_plan(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model
                   int variant, ihe_band_info *info) {
   if (!info) return fail(IHE_ERR_ARG, "null info");
   HostBand H;
-  bool wide = false;
-  if (int rc = plan_band(layout, num_in_cts, model, variant, H, wide)) return rc;
-  fill_info(H, model, wide, info);
+  if (int rc = plan_band(layout, num_in_cts, model, variant, H)) return rc;
+  fill_info(H, model, info);
   return IHE_OK;
 }
 
@@ -455,31 +903,33 @@ int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
   if (!out) return fail(IHE_ERR_ARG, "null argument");
   *out = nullptr;
   HostBand H;
-  bool wide = false;
-  if (int rc = plan_band(layout, num_in_cts, model, variant, H, wide)) return rc;
+  if (int rc = plan_band(layout, num_in_cts, model, variant, H)) return rc;
 
   ihe_band *b = new ihe_band();
+  b->wpl = (int)env_u32("IHE_CLOUD_WPL", 1, 1, 4);
+  if (b->wpl == 3) b->wpl = 2;
   b->ctx = ctx;
   b->layout = *layout;
   b->num_in = num_in_cts;
   b->num_out = model->num_outputs;
-  b->variant = wide ? IHE_VARIANT_WIDE : IHE_VARIANT_VALU;
-  b->num_groups = (uint32_t)H.grp.size();
-  b->num_tiles = (uint32_t)H.tile_off.size() - 1;
-  b->num_cols = (uint32_t)H.cols.size();
-  b->nnz = model->num_outputs ? model->row_ptr[model->num_outputs] : 0;
-  b->max_window = H.max_window;
+  ihe_band_info info;
+  fill_info(H, model, &info);
+  b->variant = (int)info.variant;
+  b->staged = H.staged;
+  b->num_groups = info.num_groups;
+  b->num_tiles = info.num_tiles;
+  b->num_cols = info.num_cols;
+  b->nnz = info.nnz;
+  b->max_window = info.max_window;
+  b->segments = H.segments;
+  b->meta_lds = H.max_stage * kStageSlice;
+  b->lds_bytes = (b->meta_lds + H.max_meta) * 4u;
 
   size_t bytes = 0;
-  bytes += align_up(sizeof(GroupDev) * std::max<size_t>(H.grp.size(), 1));
-  bytes += align_up(4 * std::max<size_t>(H.tile_off.size(), 1));
-  bytes += align_up(4 * std::max<size_t>(H.cols.size(), 1));
-  bytes += align_up(4 * std::max<size_t>(H.outs.size(), 1));
-  bytes += align_up(4 * std::max<size_t>(H.bias.size(), 1));
-  bytes += align_up(4 * std::max<size_t>(H.wts.size(), 1));
-  bytes += align_up(4 * std::max<size_t>(H.var_ptr.size(), 1));
-  bytes += align_up(4 * std::max<size_t>(H.var_ct.size(), 1));
-  bytes += align_up(4 * std::max<size_t>(H.var_w2.size(), 1));
+  for (size_t n : {H.grp.size() * 8, H.tile_off.size(), H.cols.size(), H.oslot.size(),
+                   H.obias.size(), H.wts.size(), H.tiles.size() * 8, H.stage.size(), H.meta.size(),
+                   H.var_ptr.size(), H.var_ct.size(), H.var_w2.size()})
+    bytes += align_up(4 * std::max<size_t>(n, 1));
   if (int rc = ihe_malloc(ctx, bytes, &b->dmem)) {
     delete b;
     return rc;
@@ -496,10 +946,13 @@ int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
   };
   put(&b->d_grp, H.grp);
   put(&b->d_tile_off, H.tile_off);
-  put(&b->d_cols, H.cols);
-  put(&b->d_outs, H.outs);
-  put(&b->d_bias, H.bias);
+  put(&b->d_cols, H.staged ? std::vector<uint32_t>() : H.cols);
+  put(&b->d_oslot, H.oslot);
+  put(&b->d_obias, H.obias);
   put(&b->d_wts, H.wts);
+  put(&b->d_tiles, H.tiles);
+  put(&b->d_stage, H.stage);
+  put(&b->d_meta, H.meta);
   put(&b->d_var_ptr, H.var_ptr);
   put(&b->d_var_ct, H.var_ct);
   put(&b->d_var_w2, H.var_w2);
@@ -533,6 +986,73 @@ int ihe_band_get_info(const ihe_band *band, ihe_band_info *info) {
   return IHE_OK;
 }
 
+namespace {
+
+void launch_direct(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in, int32_t *d_out) {
+  CloudParams P;
+  P.grp = band->d_grp;
+  P.tile_off = band->d_tile_off;
+  P.cols = band->d_cols;
+  P.wts = band->d_wts;
+  P.oslot = band->d_oslot;
+  P.obias = band->d_obias;
+  P.ns = band->layout.num_samples;
+  P.rs = band->layout.region_size;
+  const int wpl = band->wpl;
+  P.nslices = kCtWords / (256u * (uint32_t)wpl);
+  const bool rot = band->layout.num_regions > 1;
+  const bool wide = band->variant == IHE_VARIANT_WIDE;
+  for (const Segment &s : band->segments) {
+    P.tile_base = s.tile_begin;
+    const uint32_t in_cts = band->num_in > s.in_base ? std::min(band->num_in - s.in_base, kSegCts) : 0u;
+    P.in = in_cts ? d_in + (size_t)s.in_base * kCtWords : nullptr;
+    P.in_bytes = in_cts * kCtWords * 4u;
+    P.out = d_out + (size_t)s.out_base * kCtWords;
+    P.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
+    const dim3 grid((s.tile_end - s.tile_begin) * P.nslices), block(256);
+#define IHE_LAUNCH(W, RT, WD) hipLaunchKernelGGL((k_cloud<W, RT, WD>), grid, block, 0, ctx->stream, P)
+#define IHE_LAUNCH_W(RT, WD)                  \
+  do {                                        \
+    if (wpl == 4) IHE_LAUNCH(4, RT, WD);      \
+    else if (wpl == 2) IHE_LAUNCH(2, RT, WD); \
+    else IHE_LAUNCH(1, RT, WD);               \
+  } while (0)
+    if (wide) {
+      if (rot) IHE_LAUNCH_W(true, true);
+      else IHE_LAUNCH_W(false, true);
+    } else {
+      if (rot) IHE_LAUNCH_W(true, false);
+      else IHE_LAUNCH_W(false, false);
+    }
+#undef IHE_LAUNCH_W
+#undef IHE_LAUNCH
+  }
+}
+
+void launch_staged(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in, int32_t *d_out) {
+  StagedParams P;
+  P.tiles = band->d_tiles;
+  P.stage = band->d_stage;
+  P.meta = band->d_meta;
+  P.ns = band->layout.num_samples;
+  P.rs = band->layout.region_size;
+  P.meta_lds = band->meta_lds;
+  const bool rot = band->layout.num_regions > 1;
+  for (const Segment &s : band->segments) {
+    P.tile_base = s.tile_begin;
+    const uint32_t in_cts = band->num_in > s.in_base ? std::min(band->num_in - s.in_base, kSegCts) : 0u;
+    P.in = in_cts ? d_in + (size_t)s.in_base * kCtWords : nullptr;
+    P.in_bytes = in_cts * kCtWords * 4u;
+    P.out = d_out + (size_t)s.out_base * kCtWords;
+    P.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
+    const dim3 grid((s.tile_end - s.tile_begin) * (kCtWords / kStageSlice)), block(kStageSlice);
+    if (rot) hipLaunchKernelGGL(k_cloud_staged<true>, grid, block, band->lds_bytes, ctx->stream, P);
+    else hipLaunchKernelGGL(k_cloud_staged<false>, grid, block, band->lds_bytes, ctx->stream, P);
+  }
+}
+
+}  // namespace
+
 int ihe_cloud_score(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in,
                     const double *d_in_var, int32_t *d_out, double *d_out_var) {
   if (int rc = check_ctx(ctx)) return rc;
@@ -541,36 +1061,15 @@ int ihe_cloud_score(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in,
   if (!d_out || (band->num_in > 0 && !d_in)) return fail(IHE_ERR_ARG, "null ciphertext buffer");
   if (d_out_var && band->num_in > 0 && !d_in_var) return fail(IHE_ERR_ARG, "null input variance");
   if (int rc = make_current(ctx)) return rc;
-
-  CloudParams P;
-  P.grp = band->d_grp;
-  P.tile_off = band->d_tile_off;
-  P.cols = band->d_cols;
-  P.outs = band->d_outs;
-  P.wts = band->d_wts;
-  P.bias = band->d_bias;
-  P.in = d_in;
-  P.out = d_out;
-  P.ns = band->layout.num_samples;
-  P.rs = band->layout.region_size;
-  const int wpl = band->wpl;
-  P.nslices = kCtWords / (256u * (uint32_t)wpl);
-  const bool rot = band->layout.num_regions > 1;
-  const dim3 grid(band->num_tiles * P.nslices), block(256);
   {
     ScopedTimer t(ctx, IHE_TIMER_CLOUD);
-    if (band->variant == IHE_VARIANT_WIDE) {
-      if (rot) hipLaunchKernelGGL((k_cloud_wide<1, true>), grid, block, 0, ctx->stream, P);
-      else hipLaunchKernelGGL((k_cloud_wide<1, false>), grid, block, 0, ctx->stream, P);
-    } else {
-      if (rot) hipLaunchKernelGGL((k_cloud_dot2<1, true>), grid, block, 0, ctx->stream, P);
-      else hipLaunchKernelGGL((k_cloud_dot2<1, false>), grid, block, 0, ctx->stream, P);
-    }
+    if (band->staged) launch_staged(ctx, band, d_in, d_out);
+    else launch_direct(ctx, band, d_in, d_out);
   }
   IHE_HIP(hipGetLastError());
   if (d_out_var) {
     ScopedTimer t(ctx, IHE_TIMER_CLOUD_VAR);
-    const dim3 vgrid((band->num_out + 255) / 256);
+    const dim3 vgrid((band->num_out + 255) / 256), block(256);
     hipLaunchKernelGGL(k_cloud_var, vgrid, block, 0, ctx->stream, band->d_var_ptr, band->d_var_ct,
                        band->d_var_w2, d_in_var, d_out_var, band->num_out);
     IHE_HIP(hipGetLastError());

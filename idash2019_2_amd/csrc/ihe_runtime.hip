@@ -1,4 +1,5 @@
 // Context, device memory, errors and timing of libidash_hip.so.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -229,6 +230,40 @@ int ihe_memset(ihe_ctx *ctx, void *dst, int value, size_t bytes) {
   if (!dst) return fail(IHE_ERR_ARG, "null pointer");
   IHE_HIP(hipSetDevice(ctx->device));
   IHE_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  return IHE_OK;
+}
+
+}  // extern "C"
+
+namespace ihe {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy16(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
+                                               size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+}  // namespace ihe
+
+extern "C" {
+
+int ihe_memcpy_d2d(ihe_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (bytes == 0) return IHE_OK;
+  if (!dst || !src) return fail(IHE_ERR_ARG, "null copy pointer");
+  IHE_HIP(hipSetDevice(ctx->device));
+  if ((bytes | (uintptr_t)dst | (uintptr_t)src) % 16 != 0) {
+    IHE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return IHE_OK;
+  }
+  const size_t n16 = bytes / 16;
+  const size_t blocks = std::min<size_t>((n16 + 255) / 256, (size_t)ctx->num_cus * 32);
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                     (const u32x4 *)src, (u32x4 *)dst, n16);
+  IHE_HIP(hipGetLastError());
   return IHE_OK;
 }
 

@@ -140,6 +140,10 @@ class Engine:
     def sync(self) -> None:
         check(lib().ihe_sync(self.ctx))
 
+    def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int | None = None) -> None:
+        n = min(dst.nbytes, src.nbytes) if nbytes is None else nbytes
+        check(lib().ihe_memcpy_d2d(self.ctx, dst.ptr, src.ptr, n))
+
     # -- timing
     def timing(self, on: bool) -> None:
         check(lib().ihe_timing_enable(self.ctx, int(on)))

@@ -98,6 +98,10 @@ int ihe_memcpy_d2h(ihe_ctx *ctx, void *dst, const void *src, size_t bytes);
 int ihe_memcpy_h2d_async(ihe_ctx *ctx, void *dst, const void *src, size_t bytes);
 int ihe_memcpy_d2h_async(ihe_ctx *ctx, void *dst, const void *src, size_t bytes);
 int ihe_memset(ihe_ctx *ctx, void *dst, int value, size_t bytes);
+/* Device-to-device copy by a streaming kernel (16-byte aligned buffers and
+ * sizes; otherwise hipMemcpyAsync). Also the reference copy of known size
+ * that HBM traffic counters are calibrated against. */
+int ihe_memcpy_d2d(ihe_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---------------- timing of the library's own kernels ---------------- */
 enum {

@@ -61,8 +61,9 @@ def test_null_arguments_are_rejected():
     assert L.ihe_band_plan(None, 0, None, 0, None) == _abi.IHE_ERR_ARG
 
 
-def test_band_plan_groups_shared_windows():
+def test_band_plan_groups_shared_windows(monkeypatch):
     c = synth.generate(60, 300, 128, 5, seed=3)
+    monkeypatch.setenv("IHE_CLOUD_STAGED", "0")
     info = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
     assert info["num_outputs"] == c.num_outputs
     assert info["variant"] == _abi.VARIANT_VALU
@@ -71,7 +72,13 @@ def test_band_plan_groups_shared_windows():
     # with the same window merge until 12 outputs
     windows = np.unique(c.lo).size
     assert windows <= info["num_groups"] <= c.num_targets
-    assert info["num_cols"] == 16 * info["num_groups"]   # 15 columns padded to 8 pairs
+    assert info["num_cols"] == 16 * info["num_groups"]   # direct: 15 columns padded to 16
+    # staged: consecutive windows share their staged columns within a tile
+    monkeypatch.setenv("IHE_CLOUD_STAGED", "1")
+    staged = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
+    assert staged["num_groups"] == info["num_groups"]
+    used = np.unique(3 * c.lo[:, None] + np.arange(15)[None, :]).size
+    assert used <= staged["num_cols"] < 16 * info["num_groups"] / 2
 
 
 def test_band_plan_wide_weights_select_wide_variant():
@@ -108,4 +115,5 @@ def test_band_plan_rejects_duplicates_and_bad_layout():
 
 def test_band_plan_empty_and_bias_only_outputs():
     info = band_plan(64, 0, [0, 1, 1], [_abi.IHE_CONSTANT_BIDX], [7])
-    assert info["num_outputs"] == 2 and info["num_cols"] == 0
+    # one group without columns
+    assert info["num_outputs"] == 2 and info["num_groups"] == 1 and info["max_window"] == 0
