@@ -63,7 +63,7 @@ def test_null_arguments_are_rejected():
 
 def test_band_plan_groups_shared_windows(monkeypatch):
     c = synth.generate(60, 300, 128, 5, seed=3)
-    monkeypatch.setenv("IHE_CLOUD_STAGED", "0")
+    monkeypatch.setenv("IHE_CLOUD_KERNEL", "direct")
     info = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
     assert info["num_outputs"] == c.num_outputs
     assert info["variant"] == _abi.VARIANT_VALU
@@ -74,21 +74,37 @@ def test_band_plan_groups_shared_windows(monkeypatch):
     assert windows <= info["num_groups"] <= c.num_targets
     assert info["num_cols"] == 16 * info["num_groups"]   # direct: 15 columns padded to 16
     # staged: consecutive windows share their staged columns within a tile
-    monkeypatch.setenv("IHE_CLOUD_STAGED", "1")
+    monkeypatch.setenv("IHE_CLOUD_KERNEL", "staged")
     staged = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
     assert staged["num_groups"] == info["num_groups"]
     used = np.unique(3 * c.lo[:, None] + np.arange(15)[None, :]).size
     assert used <= staged["num_cols"] < 16 * info["num_groups"] / 2
+    # MFMA: tiles of 32 consecutive outputs over the union of their windows
+    monkeypatch.setenv("IHE_CLOUD_KERNEL", "mfma")
+    mfma = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
+    assert mfma["variant"] == _abi.VARIANT_MFMA
+    assert mfma["num_tiles"] == -(-c.num_outputs // 32)
+    assert used <= mfma["num_cols"] <= 32 * mfma["num_tiles"]
+    # default: single-K-block tiles, cut short where a union passes 32 columns
+    monkeypatch.delenv("IHE_CLOUD_KERNEL")
+    k1 = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
+    assert k1["variant"] == _abi.VARIANT_MFMA
+    assert mfma["num_tiles"] <= k1["num_tiles"] < 1.2 * mfma["num_tiles"]
+    assert used <= k1["num_cols"] <= 32 * k1["num_tiles"]
 
 
 def test_band_plan_wide_weights_select_wide_variant():
     rp = np.array([0, 2], np.uint32)
     info = band_plan(1024, 2, rp, np.array([0, 1], np.uint32), np.array([1 << 20, 3], np.int32))
     assert info["variant"] == _abi.VARIANT_WIDE
-    with pytest.raises(IheError) as e:
-        band_plan(1024, 2, rp, np.array([0, 1], np.uint32), np.array([1 << 20, 3], np.int32),
-                  variant=_abi.VARIANT_VALU)
-    assert e.value.code == _abi.IHE_ERR_UNSUPPORTED
+    for v in (_abi.VARIANT_VALU, _abi.VARIANT_MFMA):
+        with pytest.raises(IheError) as e:
+            band_plan(1024, 2, rp, np.array([0, 1], np.uint32), np.array([1 << 20, 3], np.int32),
+                      variant=v)
+        assert e.value.code == _abi.IHE_ERR_UNSUPPORTED
+    # 32640..32767 fit int16 but not the balanced two-byte split of the MFMA kernel
+    info = band_plan(1024, 2, rp, np.array([0, 1], np.uint32), np.array([32700, 3], np.int32))
+    assert info["variant"] == _abi.VARIANT_VALU
 
 
 def test_band_plan_missing_ciphertext_is_range_error():

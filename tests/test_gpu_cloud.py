@@ -9,6 +9,24 @@ from idash2019_2_amd._abi import IheError
 
 pytestmark = pytest.mark.gpu
 
+# The cloud kernels: single-K-block byte-split MFMA (the default for int16
+# weights where every 32-row tile fits 32 columns), general byte-split MFMA,
+# warp-specialized LDS-staged MFMA, LDS-staged dot2 VALU and direct dot2 VALU.
+KERNELS = ["mfma1", "mfma", "ws", "staged", "direct"]
+
+
+@pytest.fixture(params=KERNELS)
+def kernel(request, monkeypatch):
+    if request.param == "mfma1":
+        monkeypatch.delenv("IHE_CLOUD_KERNEL", raising=False)
+    else:
+        monkeypatch.setenv("IHE_CLOUD_KERNEL", request.param)
+    return request.param
+
+
+def expected_variant(kernel):
+    return _abi.VARIANT_MFMA if kernel in ("mfma1", "mfma", "ws") else _abi.VARIANT_VALU
+
 
 def run_cloud(engine, ct, var, row_ptr, in_bidx, weight, ns, variant=_abi.VARIANT_AUTO):
     num_in = ct.shape[0]
@@ -32,7 +50,7 @@ def random_cts(rng, num_in):
 
 
 @pytest.mark.parametrize("ns", [128, 1024, 100, 333, 512, 1000, 1, 7])
-def test_banded_model_matches_oracle(engine, ns):
+def test_banded_model_matches_oracle(engine, kernel, ns):
     c = synth.generate(40, 150, ns, 5, seed=ns)
     rng = np.random.default_rng(ns)
     ct = random_cts(rng, c.num_in_cts)
@@ -40,14 +58,14 @@ def test_banded_model_matches_oracle(engine, ns):
     var[::3] = 2.0 ** -40
     rp, ib, w = c.model_csr()
     out, ov, info = run_cloud(engine, ct, var, rp, ib, w, ns)
-    assert info["variant"] == _abi.VARIANT_VALU
+    assert info["variant"] == expected_variant(kernel)
     ref, rv = oracle.cloud(ct, var, rp, ib, w, *c.layout)
     assert (out == ref).all()
     assert (ov == rv).all()
 
 
 @pytest.mark.parametrize("ns", [128, 1024])
-def test_zero_weights_dropped_matches_oracle(engine, ns):
+def test_zero_weights_dropped_matches_oracle(engine, kernel, ns):
     c = synth.generate(30, 90, ns, 5, seed=7)
     c.weights[::4, 3] = 0                            # .hr files omit zero coefficients
     rng = np.random.default_rng(1)
@@ -61,6 +79,8 @@ def test_zero_weights_dropped_matches_oracle(engine, ns):
 
 @pytest.mark.parametrize("ns,variant,wmax", [(1024, _abi.VARIANT_VALU, 32767),
                                              (64, _abi.VARIANT_VALU, 32767),
+                                             (1024, _abi.VARIANT_MFMA, 32639),
+                                             (77, _abi.VARIANT_MFMA, 32639),
                                              (1024, _abi.VARIANT_WIDE, (1 << 31) - 1),
                                              (50, _abi.VARIANT_WIDE, (1 << 31) - 1),
                                              (1024, _abi.VARIANT_AUTO, 1 << 20)])
@@ -89,12 +109,14 @@ def test_random_sparse_model_matches_oracle(engine, ns, variant, wmax):
     out, ov, info = run_cloud(engine, ct, var, rp, ib, w, ns, variant)
     if wmax > 32767:
         assert info["variant"] == _abi.VARIANT_WIDE
+    else:
+        assert info["variant"] == (variant or _abi.VARIANT_MFMA)
     ref, rv = oracle.cloud(ct, var, rp, ib, w, ns, R, 1024 // R)
     assert (out == ref).all()
     assert (ov == rv).all()
 
 
-def test_wide_window_matches_oracle(engine):
+def test_wide_window_matches_oracle(engine, kernel):
     """50 neighbours (150 columns, several register chunks per group)."""
     c = synth.generate(120, 200, 1024, 50, seed=50)
     rng = np.random.default_rng(50)
@@ -127,7 +149,8 @@ def test_output_without_variance_buffer(engine):
     band.free()
 
 
-def test_full_size_rows_match_oracle(engine):
+@pytest.mark.parametrize("kernel", ["mfma1", "mfma", "staged"], indirect=True)
+def test_full_size_rows_match_oracle(engine, kernel):
     """The full 16,184-tag / 80,882-target / 1,024-sample workload on the GPU;
     a random subset of 600 output rows (plus the first and last) is recomputed
     by the oracle, and every output's b-polynomial tail and variance are checked
