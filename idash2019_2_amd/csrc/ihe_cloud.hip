@@ -638,15 +638,17 @@ __global__ __launch_bounds__(256) void k_cloud_mfma(MfmaParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// Single-K-block MFMA kernel: tiles of at most 32 columns (every 32-row tile of
-// a 5-neighbour band fits). A wave keeps the tile's A operands in registers and
-// walks NTW column tiles of 32 words. Every step issues the same vector-memory
-// sequence, 16 loads (the next step's columns; the last step reloads its own)
-// followed by 16 stores, so the wait for the loads is vmcnt(16) and the output
-// stores drain while the next step computes. Rows past the tile's end store to
-// an offset beyond the buffer's range, which the hardware drops. Row biases and
-// the per-lane column offsets live in LDS. Blocks are renumbered so that each
-// XCD (blocks are dealt round-robin over the 8 XCDs) walks a contiguous run of
+// Single-K-block MFMA kernel: tiles of 32*RB rows over at most 32 columns (a
+// 5-neighbour band fits 64-row tiles almost everywhere). A wave keeps the
+// tile's A operands in registers and walks NTW column tiles of 32 words; each
+// step's B operand (16 loads per lane) feeds RB row blocks of 32 outputs, so
+// the input words read per output word fall to 1/RB. Every step issues the
+// same vector-memory sequence, 16 loads (the next step's columns; the last
+// step reloads its own) followed by 16*RB stores, so the wait for the loads
+// leaves the stores in flight while the next step computes. Rows past the
+// tile's end store to an offset beyond the buffer's range, which the hardware
+// drops. Row biases live in LDS. Blocks are renumbered so that each XCD
+// (blocks are dealt round-robin over the 8 XCDs) walks a contiguous run of
 // tiles, whose windows overlap in that XCD's L2.
 constexpr uint32_t kK1Waves = 4;
 constexpr uint32_t kDropOffset = 0x80000000u;  // >= every segment's byte size
@@ -672,10 +674,11 @@ __device__ __forceinline__ void k1_load(__amdgpu_buffer_rsrc_t in_r, const uint3
   }
 }
 
-template <bool ROT, int NTW>
+template <bool ROT, int NTW, int RB>
 __global__ __launch_bounds__(64 * kK1Waves) void k_cloud_mfma1(MfmaParams P) {
   constexpr uint32_t wg_per_tile = kCtWords / (kK1Waves * NTW * 32);
-  __shared__ uint32_t s_bias[kMfmaRows];
+  constexpr uint32_t rows = kMfmaRows * RB;
+  __shared__ uint32_t s_bias[rows];
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t tile = P.tile_base + blk / wg_per_tile;
   const uint32_t part = blk % wg_per_tile;
@@ -686,7 +689,7 @@ __global__ __launch_bounds__(64 * kK1Waves) void k_cloud_mfma1(MfmaParams P) {
       __builtin_amdgcn_make_buffer_rsrc((void *)P.in, (short)0, (int)P.in_bytes, kRsrcWord3);
   const __amdgpu_buffer_rsrc_t out_r =
       __builtin_amdgcn_make_buffer_rsrc((void *)P.out, (short)0, (int)P.out_bytes, kRsrcWord3);
-  if (threadIdx.x < kMfmaRows) s_bias[threadIdx.x] = P.rowc[T.row_off + threadIdx.x];
+  if (threadIdx.x < rows) s_bias[threadIdx.x] = P.rowc[T.row_off + threadIdx.x];
   // the lane's 16 column words: columns 16*h + j
   uint32_t cw[16];
   {
@@ -696,67 +699,70 @@ __global__ __launch_bounds__(64 * kK1Waves) void k_cloud_mfma1(MfmaParams P) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) cw[j] = h ? chh[j] : cl[j];
   }
-  const int4v a0 = P.adata[(size_t)T.a_off * 64 + lane];
-  const int4v a1 = P.adata[(size_t)(T.a_off + 1) * 64 + lane];
-  const uint32_t nt0 = (part * kK1Waves + wave) * NTW;  // first column tile of this wave
-  // store offsets: lane part (row half, word) and per-value row offsets; rows
-  // at or past nrows are sent out of range
-  const uint32_t row_base = T.slot0 * kCtWords * 4u;
-  uint32_t vrow[4];
+  int4v a[RB][2];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const uint32_t m = 8 * g + 4 * h;  // rows m .. m+3 for values 4g .. 4g+3
-    vrow[g] = 4 * h * kCtWords * 4u + (m < T.nrows ? 0u : kDropOffset);
+  for (int rb = 0; rb < RB; ++rb) {
+    a[rb][0] = P.adata[(size_t)(T.a_off + 2 * rb) * 64 + lane];
+    a[rb][1] = P.adata[(size_t)(T.a_off + 2 * rb + 1) * 64 + lane];
   }
-  const bool full = T.nrows == kMfmaRows;
+  const uint32_t nt0 = (part * kK1Waves + wave) * NTW;  // first column tile of this wave
+  // store offsets: the lane's row half, plus an out-of-range offset for rows at
+  // or past nrows; per-value row offsets are uniform
+  const uint32_t row_base = T.slot0 * kCtWords * 4u;
+  const bool full = T.nrows == rows;
   __syncthreads();
 
   // one step: column tile nt from x, stores issued last
   auto step = [&](const uint32_t (&x)[16], uint32_t nt) {
     int4v pl[4];
     byte_planes(x, pl);
-    int16v acc0 = {}, acc1 = {}, acc2 = {}, acc3 = {};
-    acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[0], acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[1], acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[2], acc2, 0, 0, 0);
-    acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[3], acc3, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[0], acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[1], acc2, 0, 0, 0);
-    acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[2], acc3, 0, 0, 0);
     const uint32_t n0 = (nt0 + nt) * 32;  // uniform
     const uint32_t word = n0 + r;
-    uint32_t o[16];
-#pragma unroll
-    for (int v = 0; v < 16; ++v)
-      o[v] = (uint32_t)acc0[v] + ((uint32_t)acc1[v] << 8) + ((uint32_t)acc2[v] << 16) +
-             ((uint32_t)acc3[v] << 24);
+    const uint32_t voff = word * 4u + 4 * h * kCtWords * 4u;
     // b polynomial: bias on samples < NS, words >= RS hidden; both uniform per
     // column tile except where NS or RS falls inside it
     const bool b_poly = n0 >= kN;
-    if (b_poly && n0 - kN < P.ns) {
-      const bool lane_bias = word - kN < P.ns;
+    const bool bias_tile = b_poly && n0 - kN < P.ns, lane_bias = word - kN < P.ns;
+    const bool hide_tile = b_poly && n0 + 31 - kN >= P.rs, keep = word - kN < P.rs;
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const u32x4 bv = *(const u32x4 *)(s_bias + 8 * g + 4 * h);
+    for (int rb = 0; rb < RB; ++rb) {
+      int16v acc0 = {}, acc1 = {}, acc2 = {}, acc3 = {};
+      acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][0], pl[0], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][0], pl[1], acc1, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][0], pl[2], acc2, 0, 0, 0);
+      acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][0], pl[3], acc3, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][1], pl[0], acc1, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][1], pl[1], acc2, 0, 0, 0);
+      acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][1], pl[2], acc3, 0, 0, 0);
+      uint32_t o[16];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) o[4 * g + q] += lane_bias ? bv[q] : 0u;
+      for (int v = 0; v < 16; ++v)
+        o[v] = (uint32_t)acc0[v] + ((uint32_t)acc1[v] << 8) + ((uint32_t)acc2[v] << 16) +
+               ((uint32_t)acc3[v] << 24);
+      if (bias_tile) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const u32x4 bv = *(const u32x4 *)(s_bias + 32 * rb + 8 * g + 4 * h);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) o[4 * g + q] += lane_bias ? bv[q] : 0u;
+        }
+      }
+      if (hide_tile) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) o[v] = keep ? o[v] : 0u;
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const uint32_t m = 32 * rb + 8 * (v / 4) + (v % 4);  // row of lanes 0..31
+        __builtin_amdgcn_raw_buffer_store_b32(
+            (int)o[v], out_r, full || m + 4 * h < T.nrows ? voff : kDropOffset,
+            (int)(row_base + m * kCtWords * 4u), IHE_STORE_POLICY);
       }
     }
-    if (b_poly && n0 + 31 - kN >= P.rs) {
-      const bool keep = word - kN < P.rs;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) o[v] = keep ? o[v] : 0u;
-    }
-    const uint32_t voff = word * 4u;
-#pragma unroll
-    for (int v = 0; v < 16; ++v)
-      __builtin_amdgcn_raw_buffer_store_b32(
-          (int)o[v], out_r, voff + (full ? 4 * h * kCtWords * 4u : vrow[v / 4]),
-          (int)(row_base + (8 * (v / 4) + (v % 4)) * kCtWords * 4u), IHE_STORE_POLICY);
   };
 
   // The first step is peeled so that the loop is entered, like it is re-entered,
-  // with 16 loads followed by 16 stores outstanding.
+  // with 16 loads followed by the stores outstanding.
   uint32_t x[16], xn[16];
   k1_load<ROT>(in_r, cw, nt0 * 32 + r, x);
   k1_load<ROT>(in_r, cw, (nt0 + (NTW > 1 ? 1 : 0)) * 32 + r, xn);
@@ -990,6 +996,7 @@ struct ihe_band {
   int variant = IHE_VARIANT_VALU;
   int mode = 0;  // kModeDirect, kModeStaged or kModeMfma
   int wpl = 1;
+  uint32_t row_blocks = 1;  // 32-row MFMA blocks per tile
   uint32_t num_groups = 0, num_tiles = 0, num_cols = 0, nnz = 0, max_window = 0;
   uint32_t lds_bytes = 0, meta_lds = 0, max_kb = 0;
   size_t adata_bytes = 0, rowc_bytes = 0;
@@ -1037,7 +1044,7 @@ struct HostBand {
   std::vector<MTileDev> mtiles;
   std::vector<uint32_t> kcols, rowc;
   std::vector<int4v> adata;
-  uint32_t max_kb = 0;
+  uint32_t max_kb = 0, row_blocks = 1;
   // all layouts
   std::vector<Segment> segments;
   std::vector<uint32_t> var_ptr, var_ct, bias;
@@ -1348,7 +1355,9 @@ void layout_staged(const ihe_layout &L, uint32_t tile_outputs, HostBand &H) {
 
 // MFMA layout: tiles of 32 consecutive output slots over the union of their
 // columns. Returns false when a tile's union exceeds kMfmaMaxKb blocks.
-bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, HostBand &H) {
+bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, uint32_t row_blocks,
+                 HostBand &H) {
+  const uint32_t rows = kMfmaRows * row_blocks;
   const uint32_t R = L.num_regions, RS = L.region_size;
   // output slot -> (group, row within the group)
   std::vector<uint32_t> og(num_out), orow(num_out);
@@ -1357,7 +1366,7 @@ bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, Host
       og[H.gout[H.hg[g].out_begin + i]] = g;
       orow[H.gout[H.hg[g].out_begin + i]] = i;
     }
-  // tiles: up to 32 consecutive rows whose column union fits kb_budget blocks
+  // tiles: up to `rows` consecutive rows whose column union fits kb_budget blocks
   std::vector<std::vector<uint32_t>> U;
   std::vector<uint32_t> t_o0, t_rows;
   std::vector<uint32_t> merged;
@@ -1365,7 +1374,7 @@ bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, Host
     const uint32_t o0 = o;
     std::vector<uint32_t> u;
     uint32_t last_g = UINT32_MAX;
-    while (o < num_out && o - o0 < kMfmaRows) {
+    while (o < num_out && o - o0 < rows) {
       const uint32_t g = og[o];
       if (g != last_g) {
         const HostGroup &hg = H.hg[g];
@@ -1413,7 +1422,7 @@ bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, Host
       for (uint32_t k = 0; k < 32 * nkb; ++k)
         H.kcols.push_back(col_word(k < u.size() ? u[k] : fill, R, RS, seg.in_base));
       // dense weights of the tile: W[m][k]
-      std::vector<int32_t> W((size_t)kMfmaRows * 32 * nkb, 0);
+      std::vector<int32_t> W((size_t)rows * 32 * nkb, 0);
       for (uint32_t m = 0; m < nrows; ++m) {
         const HostGroup &hg = H.hg[og[o0 + m]];
         const uint32_t *gc = H.gcol.data() + hg.col_begin;
@@ -1423,20 +1432,23 @@ bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, Host
           W[(size_t)m * 32 * nkb + k] = gw[i];
         }
       }
+      // A operand blocks in (row block, K-block, weight byte) order
+      for (uint32_t rb = 0; rb < row_blocks; ++rb)
       for (uint32_t kb = 0; kb < nkb; ++kb)
         for (uint32_t hp = 0; hp < 2; ++hp)
           for (uint32_t lane = 0; lane < 64; ++lane) {
             int4v a;
             int8_t bytes[16];
             for (uint32_t j = 0; j < 16; ++j) {
-              const int32_t w = W[(size_t)(lane % 32) * 32 * nkb + 32 * kb + 16 * (lane / 32) + j];
+              const uint32_t m = 32 * rb + lane % 32;
+              const int32_t w = W[(size_t)m * 32 * nkb + 32 * kb + 16 * (lane / 32) + j];
               const int32_t w0 = (int8_t)(w & 0xFF);
               bytes[j] = (int8_t)(hp == 0 ? w0 : (w - w0) / 256);
             }
             std::memcpy(&a, bytes, 16);
             H.adata.push_back(a);
           }
-      for (uint32_t m = 0; m < kMfmaRows; ++m) H.rowc.push_back(m < nrows ? H.bias[o0 + m] : 0u);
+      for (uint32_t m = 0; m < rows; ++m) H.rowc.push_back(m < nrows ? H.bias[o0 + m] : 0u);
       H.mtiles.push_back(D);
       H.max_kb = std::max(H.max_kb, nkb);
       H.cols.resize(H.cols.size() + u.size());  // counts tile columns for the band info
@@ -1515,18 +1527,20 @@ int plan_band(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr
     H.max_kb = 0;
   };
   // single-K-block tiles, else tiles within two K-blocks for the staged kernel
-  if (mode == kModeMfma1 && !layout_mfma(*layout, model->num_outputs, 1, H)) {
+  if (mode == kModeMfma1) H.row_blocks = env_u32("IHE_CLOUD_ROW_BLOCKS", 2, 1, 2);
+  if (mode == kModeMfma1 && !layout_mfma(*layout, model->num_outputs, 1, H.row_blocks, H)) {
     clear_mfma();
     mode = kModeMfma;
   }
   if (mode == kModeMfmaWs) {
     const uint32_t budget = H.max_window <= 32 ? 1 : H.max_window <= 64 ? 2 : 0;
-    if (budget == 0 || !layout_mfma(*layout, model->num_outputs, budget, H)) {
+    if (budget == 0 || !layout_mfma(*layout, model->num_outputs, budget, 1, H)) {
       clear_mfma();
       mode = kModeMfma;
     }
   }
-  if (mode == kModeMfma && !layout_mfma(*layout, model->num_outputs, kMfmaMaxKb, H)) {
+  if (mode == kModeMfma) H.row_blocks = 1;
+  if (mode == kModeMfma && !layout_mfma(*layout, model->num_outputs, kMfmaMaxKb, 1, H)) {
     if (variant == IHE_VARIANT_MFMA)
       return fail(IHE_ERR_UNSUPPORTED, "an MFMA tile needs more than 512 distinct columns");
     clear_mfma();
@@ -1600,6 +1614,7 @@ int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
   b->segments = H.segments;
   b->meta_lds = H.max_stage * kStageSlice;
   b->max_kb = H.max_kb;
+  b->row_blocks = H.row_blocks;
   b->lds_bytes = (b->meta_lds + H.max_meta) * 4u;
 
   size_t bytes = 0;
@@ -1781,10 +1796,12 @@ void launch_mfma1(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in, int32
     P.out = d_out + (size_t)s.out_base * kCtWords;
     P.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
     const dim3 grid((s.tile_end - s.tile_begin) * wg_per_tile), block(64 * kK1Waves);
-#define IHE_LAUNCH(RT, W) hipLaunchKernelGGL((k_cloud_mfma1<RT, W>), grid, block, 0, ctx->stream, P)
-    if (ntw == 16) { if (rot) IHE_LAUNCH(true, 16); else IHE_LAUNCH(false, 16); }
-    else if (ntw == 8) { if (rot) IHE_LAUNCH(true, 8); else IHE_LAUNCH(false, 8); }
-    else { if (rot) IHE_LAUNCH(true, 4); else IHE_LAUNCH(false, 4); }
+#define IHE_LAUNCH(RT, W, RB) hipLaunchKernelGGL((k_cloud_mfma1<RT, W, RB>), grid, block, 0, ctx->stream, P)
+#define IHE_LAUNCH_RB(RT, W) if (band->row_blocks == 2) IHE_LAUNCH(RT, W, 2); else IHE_LAUNCH(RT, W, 1)
+    if (ntw == 16) { if (rot) IHE_LAUNCH_RB(true, 16); else IHE_LAUNCH_RB(false, 16); }
+    else if (ntw == 8) { if (rot) IHE_LAUNCH_RB(true, 8); else IHE_LAUNCH_RB(false, 8); }
+    else { if (rot) IHE_LAUNCH_RB(true, 4); else IHE_LAUNCH_RB(false, 4); }
+#undef IHE_LAUNCH_RB
 #undef IHE_LAUNCH
   }
 }

@@ -85,11 +85,12 @@ def test_band_plan_groups_shared_windows(monkeypatch):
     assert mfma["variant"] == _abi.VARIANT_MFMA
     assert mfma["num_tiles"] == -(-c.num_outputs // 32)
     assert used <= mfma["num_cols"] <= 32 * mfma["num_tiles"]
-    # default: single-K-block tiles, cut short where a union passes 32 columns
+    # default: single-K-block tiles of up to 64 rows, cut short where the
+    # union passes 32 columns
     monkeypatch.delenv("IHE_CLOUD_KERNEL")
     k1 = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
     assert k1["variant"] == _abi.VARIANT_MFMA
-    assert mfma["num_tiles"] <= k1["num_tiles"] < 1.2 * mfma["num_tiles"]
+    assert -(-c.num_outputs // 64) <= k1["num_tiles"] <= mfma["num_tiles"]
     assert used <= k1["num_cols"] <= 32 * k1["num_tiles"]
 
 
