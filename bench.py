@@ -72,6 +72,9 @@ def main(argv=None):
     ap.add_argument("--variant", default="auto", choices=sorted(VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (a kernel of known "
+                         "traffic for scaling HBM counters) and exit")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -79,6 +82,17 @@ def main(argv=None):
               "shards by target range with no exchange, and multi-GPU runs are not part of "
               "this build", file=sys.stderr)
         return 2
+
+    if args.calibrate_copy:
+        with Engine(0) as eng:
+            n = 1 << 30
+            a, b = eng.empty((n,), np.uint8), eng.empty((n,), np.uint8)
+            a.fill_zero()
+            for _ in range(3):
+                eng.copy(b, a)
+            eng.sync()
+        log("copied 3 x 1 GiB device to device")
+        return 0
 
     t0 = time.time()
     c = synth.workload(args.workload)
