@@ -135,7 +135,8 @@ struct EncryptParams {
 };
 
 // One wavefront per ciphertext c (FeatIndex):
-//   a = Philox stream 0, b = a*s + e + one-hot messages (setScore).
+//   a = Philox stream 0, b = a*s + e + one-hot messages (setScore); without
+//   genotypes (geno == nullptr) the ciphertexts encrypt zero (tLweSymEncryptZero).
 __global__ __launch_bounds__(64 * kWavesPerBlock) void k_encrypt(EncryptParams P, KeyBits key) {
   constexpr int J = 16;
   __shared__ int32_t lds_all[kWavesPerBlock][lds_words<J>()];
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_encrypt(EncryptParams P
     const uint32_t j = jb + u;
     const uint32_t r = j / P.rs, s = j - r * P.rs;
     uint32_t v = acc[u];
-    if (r < P.R && s < P.ns) {
+    if (P.geno && r < P.R && s < P.ns) {
       const uint32_t bidx = ct * P.R + r;
       if (bidx < P.num_cols) {
         const uint32_t tag = bidx / 3, val = bidx - 3 * tag;
@@ -235,7 +236,7 @@ int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint
       layout->region_size != kN / layout->num_regions)
     return fail(IHE_ERR_ARG, "layout must satisfy R = N / NS and RS = N / R (eval/idash.cpp:409-410)");
   if (num_tags == 0) return IHE_OK;
-  if (!d_geno || !d_out || !d_out_var) return fail(IHE_ERR_ARG, "null buffer");
+  if (!d_out || !d_out_var) return fail(IHE_ERR_ARG, "null buffer");
   if (num_tags > 0x55555555u) return fail(IHE_ERR_ARG, "too many tags");
   if (int rc = make_current(ctx)) return rc;
   EncryptParams P;

@@ -1,0 +1,829 @@
+// idash-compatible host functions (eval/idash.cpp) over the gfx950 engine.
+//
+// The file formats, index assignment and error behaviour follow the
+// reference function cited at each definition. The ciphertext arithmetic
+// (encrypt, cloud_compute_score, decrypt) runs on the GPU through
+// include/idash_hip.h; a missing GPU is an error, never a CPU fallback.
+#include "idash.h"
+
+#include <sys/resource.h>
+#include <sys/time.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <random>
+#include <sstream>
+
+#include "idash_hip.h"
+#include "parse_vw.h"
+
+using std::string;
+
+// ---------------------------------------------------------------------------
+// constants (eval/idash.cpp:20-45)
+
+Torus32 dtot32(double d) {
+  const double frac = d - (double)(int64_t)d;
+  return (Torus32)(int64_t)(frac * 4294967296.0);
+}
+double t32tod(Torus32 x) { return (double)x / 4294967296.0; }
+
+namespace {
+const TLweParams kTlweParams = {1024, 1, 0x1p-25, 0.25};
+}
+
+const double IdashParams::alpha = 0x1p-25;
+const TLweParams *IdashParams::tlweParams = &kTlweParams;
+const double IdashParams::IN_SCALING_FACTOR = 1.0 / 16384;
+const double IdashParams::ONE_IN_D = IdashParams::IN_SCALING_FACTOR;
+const double IdashParams::NAN_0_IN_D = 3. / 6. * IdashParams::IN_SCALING_FACTOR;
+const double IdashParams::NAN_1_IN_D = 2. / 6. * IdashParams::IN_SCALING_FACTOR;
+const double IdashParams::NAN_2_IN_D = 1. / 6. * IdashParams::IN_SCALING_FACTOR;
+const Torus32 IdashParams::ONE_IN_T32 = dtot32(IdashParams::ONE_IN_D);
+const Torus32 IdashParams::NAN_0_IN_T32 = dtot32(IdashParams::NAN_0_IN_D);
+const Torus32 IdashParams::NAN_1_IN_T32 = dtot32(IdashParams::NAN_1_IN_D);
+const Torus32 IdashParams::NAN_2_IN_T32 = dtot32(IdashParams::NAN_2_IN_D);
+
+IdashKey::IdashKey(const IdashParams *p, const TLweKey *k) : idashParams(p), tlweKey(k) {}
+
+// ---------------------------------------------------------------------------
+// engine access
+
+namespace {
+
+constexpr size_t kCtWords = 2 * IdashParams::N;
+constexpr size_t kCtBytes = kCtWords * sizeof(int32_t);
+// One sample in a file: 4-byte framing word, a, b, variance (8204 bytes).
+constexpr size_t kSampleStreamBytes = 4 + kCtBytes + sizeof(double);
+constexpr int32_t kSampleStreamTag = 0;
+
+[[noreturn]] void die_engine(const char *what) {
+  std::cout << "ERROR: " << what << ": " << ihe_last_error() << std::endl;
+  abort();
+}
+
+void check(int rc, const char *what) {
+  if (rc != IHE_OK) die_engine(what);
+}
+
+// Process-wide context on device IHE_DEVICE (default 0), created on first use.
+ihe_ctx *engine() {
+  static ihe_ctx *ctx = nullptr;
+  if (!ctx) {
+    const char *dev = std::getenv("IHE_DEVICE");
+    check(ihe_ctx_create(dev ? std::atoi(dev) : 0, &ctx), "no usable gfx950 device");
+  }
+  return ctx;
+}
+
+// The context if a device is present, without failing when there is none.
+ihe_ctx *engine_if_present() {
+  static int present = -1;
+  if (present < 0) {
+    int n = 0;
+    present = ihe_device_count(&n) == IHE_OK && n > 0 ? 1 : 0;
+  }
+  return present ? engine() : nullptr;
+}
+
+// Device buffer released at scope exit.
+struct DeviceBuffer {
+  void *p = nullptr;
+  explicit DeviceBuffer(size_t bytes) {
+    if (bytes) check(ihe_malloc(engine(), bytes, &p), "device allocation");
+  }
+  ~DeviceBuffer() {
+    if (p) ihe_free(engine(), p);
+  }
+  template <typename T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+struct DeviceKey {
+  ihe_key *k = nullptr;
+  explicit DeviceKey(const TLweKey *key) { check(ihe_key_create(engine(), key->key, &k), "key upload"); }
+  ~DeviceKey() { ihe_key_destroy(k); }
+};
+
+ihe_layout layout_of(const IdashParams &p) { return {p.NUM_SAMPLES, p.NUM_REGIONS, p.REGION_SIZE}; }
+
+uint64_t fresh_seed() {
+  if (const char *s = std::getenv("IHE_SEED")) {
+    static uint64_t next = std::strtoull(s, nullptr, 0);
+    return next++;
+  }
+  std::random_device rd;
+  return ((uint64_t)rd() << 32) ^ rd();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// sample storage
+
+SampleBlock::SampleBlock(size_t count) : count_(count), views_(new TLweSample[count ? count : 1]) {
+  const size_t wbytes = std::max<size_t>(count * kCtBytes, 64);
+  const size_t vbytes = std::max<size_t>(count * sizeof(double), 64);
+  if (ihe_ctx *ctx = engine_if_present()) {
+    void *w = nullptr, *v = nullptr;
+    check(ihe_host_alloc(ctx, wbytes, &w), "pinned host allocation");
+    check(ihe_host_alloc(ctx, vbytes, &v), "pinned host allocation");
+    words_ = static_cast<int32_t *>(w);
+    var_ = static_cast<double *>(v);
+    pinned_ = true;
+  } else {
+    words_ = static_cast<int32_t *>(std::aligned_alloc(4096, (wbytes + 4095) & ~size_t(4095)));
+    var_ = static_cast<double *>(std::aligned_alloc(64, (vbytes + 63) & ~size_t(63)));
+    REQUIRE_DRAMATICALLY(words_ && var_, "host allocation failed");
+  }
+  for (size_t i = 0; i < count; ++i) {
+    views_[i].a = words_ + i * kCtWords;
+    views_[i].b = words_ + i * kCtWords + IdashParams::N;
+    views_[i].current_variance = var_ + i;
+  }
+}
+
+SampleBlock::~SampleBlock() {
+  if (pinned_) {
+    ihe_host_free(engine(), words_);
+    ihe_host_free(engine(), var_);
+  } else {
+    std::free(words_);
+    std::free(var_);
+  }
+}
+
+namespace {
+
+// Dense device-ready copy of `map` (slot i = key i, i < count): the block the
+// map already points into when it is laid out that way, else a gathered copy.
+template <typename Map>
+std::shared_ptr<SampleBlock> dense_view(const Map &map, const std::shared_ptr<SampleBlock> &block,
+                                        size_t count) {
+  if (block && block->size() == count) {
+    bool dense = true;
+    for (const auto &e : map)
+      if (e.first >= count || e.second != block->sample(e.first)) {
+        dense = false;
+        break;
+      }
+    if (dense) return block;
+  }
+  auto out = std::make_shared<SampleBlock>(count);
+  std::memset(out->words(), 0, count * kCtBytes);
+  std::memset(out->variances(), 0, count * sizeof(double));
+  for (const auto &e : map) {
+    REQUIRE_DRAMATICALLY(e.first < count, "ciphertext index out of range");
+    std::memcpy(out->sample(e.first)->a, e.second->a, IdashParams::N * sizeof(int32_t));
+    std::memcpy(out->sample(e.first)->b, e.second->b, IdashParams::N * sizeof(int32_t));
+    *out->sample(e.first)->current_variance = *e.second->current_variance;
+  }
+  return out;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// parameters (eval/idash.cpp:128-246)
+
+namespace {
+
+template <typename T>
+void put(std::ostream &out, const T &v) {
+  out.write(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+template <typename T>
+void get(std::istream &in, T &v) {
+  in.read(reinterpret_cast<char *>(&v), sizeof(T));
+}
+
+void write_params_stream(const IdashParams &p, std::ostream &out) {
+  put(out, p.NUM_SAMPLES);
+  put(out, p.NUM_INPUT_POSITIONS);
+  put(out, p.NUM_OUTPUT_POSITIONS);
+  put(out, p.NUM_INPUT_FEATURES);
+  put(out, p.NUM_OUTPUT_FEATURES);
+  put(out, p.NUM_REGIONS);
+  put(out, p.REGION_SIZE);
+  REQUIRE_DRAMATICALLY(p.NUM_INPUT_POSITIONS == p.in_features_index.size(),
+                       "NUM_INPUT_POSITIONS != in_features_index.size()");
+  for (const auto &e : p.in_features_index) {
+    put(out, e.first);
+    for (FeatBigIndex b : e.second) put(out, b);
+  }
+  REQUIRE_DRAMATICALLY(p.NUM_OUTPUT_POSITIONS == p.out_features_index.size(),
+                       "NUM_OUTPUT_POSITIONS != out_features_index.size()");
+  REQUIRE_DRAMATICALLY(p.NUM_OUTPUT_POSITIONS == p.out_position_names.size(),
+                       "NUM_OUTPUT_POSITIONS != out_position_names.size()");
+  for (const auto &e : p.out_position_names) {
+    put(out, e.first);
+    out.write(e.second.c_str(), (std::streamsize)e.second.size() + 1);
+    for (FeatBigIndex b : p.out_features_index.at(e.first)) put(out, b);
+  }
+}
+
+void read_params_stream(IdashParams &p, std::istream &in) {
+  get(in, p.NUM_SAMPLES);
+  get(in, p.NUM_INPUT_POSITIONS);
+  get(in, p.NUM_OUTPUT_POSITIONS);
+  get(in, p.NUM_INPUT_FEATURES);
+  get(in, p.NUM_OUTPUT_FEATURES);
+  get(in, p.NUM_REGIONS);
+  get(in, p.REGION_SIZE);
+  for (uint32_t i = 0; i < p.NUM_INPUT_POSITIONS; ++i) {
+    uint64_t pos;
+    get(in, pos);
+    std::array<FeatBigIndex, 3> &b = p.in_features_index.emplace(pos, std::array<FeatBigIndex, 3>()).first->second;
+    for (int j = 0; j < 3; ++j) get(in, b[j]);
+  }
+  REQUIRE_DRAMATICALLY(p.NUM_INPUT_POSITIONS == p.in_features_index.size(),
+                       "NUM_INPUT_POSITIONS != in_features_index.size()");
+  for (uint32_t i = 0; i < p.NUM_OUTPUT_POSITIONS; ++i) {
+    uint64_t pos;
+    get(in, pos);
+    string name;
+    for (int c = in.get(); c > 0; c = in.get()) name.push_back((char)c);
+    p.out_position_names.push_back({pos, name});
+    std::array<FeatBigIndex, 3> &b = p.out_features_index.emplace(pos, std::array<FeatBigIndex, 3>()).first->second;
+    for (int j = 0; j < 3; ++j) get(in, b[j]);
+  }
+  REQUIRE_DRAMATICALLY(p.NUM_OUTPUT_POSITIONS == p.out_features_index.size(),
+                       "NUM_OUTPUT_POSITIONS != out_features_index.size()");
+  REQUIRE_DRAMATICALLY(p.NUM_OUTPUT_POSITIONS == p.out_position_names.size(),
+                       "NUM_OUTPUT_POSITIONS != out_position_names.size()");
+}
+
+}  // namespace
+
+void write_params(const IdashParams &params, const string &filename) {
+  std::ofstream out(filename, std::ios::binary);
+  REQUIRE_DRAMATICALLY(out.is_open(), "Cannot open parameters file for write");
+  write_params_stream(params, out);
+}
+
+void read_params(IdashParams &params, const string &filename) {
+  std::ifstream in(filename, std::ios::binary);
+  REQUIRE_DRAMATICALLY(in.is_open(), "Cannot open parameters file for read");
+  read_params_stream(params, in);
+}
+
+// ---------------------------------------------------------------------------
+// keys (eval/idash.cpp:476-502). The key stream after the parameters is this
+// build's own: N and k (int32), alpha_min and alpha_max (double), then the
+// N binary key coefficients (int32).
+
+void write_key(const IdashKey &key, const string &filename) {
+  std::ofstream out(filename, std::ios::binary);
+  REQUIRE_DRAMATICALLY(out.is_open(), "Cannot open key file for write");
+  write_params_stream(*key.idashParams, out);
+  const TLweParams *tp = key.tlweKey->params;
+  put(out, tp->N);
+  put(out, tp->k);
+  put(out, tp->alpha_min);
+  put(out, tp->alpha_max);
+  out.write(reinterpret_cast<const char *>(key.tlweKey->key), sizeof(key.tlweKey->key));
+}
+
+void read_key(IdashKey &key, const string &filename) {
+  std::ifstream in(filename, std::ios::binary);
+  REQUIRE_DRAMATICALLY(in.is_open(), "Cannot open key file for read");
+  IdashParams *params = new IdashParams();
+  read_params_stream(*params, in);
+  TLweParams tp;
+  get(in, tp.N);
+  get(in, tp.k);
+  get(in, tp.alpha_min);
+  get(in, tp.alpha_max);
+  REQUIRE_DRAMATICALLY(in && tp.N == (int32_t)IdashParams::N && tp.k == 1, "unsupported key parameters");
+  TLweKey *k = new TLweKey();
+  k->params = IdashParams::tlweParams;
+  in.read(reinterpret_cast<char *>(k->key), sizeof(k->key));
+  REQUIRE_DRAMATICALLY(in, "truncated key file");
+  for (int32_t c : k->key) REQUIRE_DRAMATICALLY(c == 0 || c == 1, "key is not binary");
+  key.idashParams = params;
+  key.tlweKey = k;
+}
+
+// ---------------------------------------------------------------------------
+// model (eval/idash.cpp:47-114)
+
+namespace {
+std::pair<uint64_t, uint8_t> split_pos_snp(const string &s) {
+  const size_t u = s.find('_');
+  return {std::stoull(s.substr(0, u)), (uint8_t)std::stoi(s.substr(u + 1))};
+}
+}  // namespace
+
+void read_model(Model &model, const IdashParams &params, const string &path) {
+  for (const auto &e : params.out_features_index) {
+    const uint64_t pos = e.first;
+    for (int snp = 0; snp < 3; ++snp) {
+      const string file = path + "/" + std::to_string(pos) + "_" + std::to_string(snp) + ".hr";
+      const std::unordered_map<string, int32_t> coefs = read(file);
+      std::unordered_map<FeatBigIndex, int32_t> &m = model.model[e.second[snp]] =
+          std::unordered_map<FeatBigIndex, int32_t>();
+      for (const auto &c : coefs) {
+        if (c.first == "Constant") {
+          m[params.constant_bigIndex()] = c.second;
+        } else {
+          const auto ps = split_pos_snp(c.first);
+          m[params.inBigIdx(ps.first, ps.second)] = c.second;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plaintext tag file (eval/idash.cpp:252-286)
+
+void read_plaintext_data(PlaintextData &plaintext_data, const IdashParams &params,
+                         const string &filename) {
+  std::ifstream in(filename);
+  REQUIRE_DRAMATICALLY(in, "file not found");
+  string line;
+  uint64_t read_positions = 0;
+  while (read_positions < params.NUM_INPUT_POSITIONS && std::getline(in, line)) {
+    std::istringstream iss(line);
+    int chrom = 0;
+    uint64_t pos, pos2;
+    string name;
+    iss >> chrom;
+    REQUIRE_DRAMATICALLY(chrom == 22, "file format error");
+    iss >> pos >> pos2 >> name;
+    REQUIRE_DRAMATICALLY(iss, "file format error");
+    std::vector<int8_t> &v = plaintext_data.data[pos];
+    v.resize(params.NUM_SAMPLES);
+    string g;
+    for (uint32_t s = 0; s < params.NUM_SAMPLES; ++s) {
+      iss >> g;
+      v[s] = (g == "0" || g == "1" || g == "2") ? (int8_t)(g[0] - '0') : (int8_t)-1;
+    }
+    ++read_positions;
+  }
+  REQUIRE_DRAMATICALLY(read_positions == params.NUM_INPUT_POSITIONS,
+                       "missing positions in plaintext file");
+}
+
+// ---------------------------------------------------------------------------
+// key generation (eval/idash.cpp:315-434)
+
+IdashParams *keygen_ph1(const string &targetFile, const string &challengeFile,
+                        const bool targetFilePosOnly) {
+  IdashParams *p = new IdashParams();
+  p->NUM_OUTPUT_POSITIONS = 0;
+  p->NUM_OUTPUT_FEATURES = 0;
+  string line;
+  uint64_t last_target = 0;
+  {
+    std::ifstream target(targetFile);
+    while (std::getline(target, line)) {
+      std::istringstream iss(line);
+      uint64_t pos;
+      string name;
+      if (targetFilePosOnly) {
+        iss >> pos;
+        name = std::to_string(pos);
+        REQUIRE_DRAMATICALLY(iss, "file format error");
+      } else {
+        int chrom = 0;
+        uint64_t pos2;
+        iss >> chrom;
+        REQUIRE_DRAMATICALLY(chrom == 22, "file format error");
+        iss >> pos >> pos2 >> name;
+        REQUIRE_DRAMATICALLY(pos2 == pos + 1, "file format error");
+        REQUIRE_DRAMATICALLY(iss, "file format error");
+      }
+      for (uint64_t snp = 0; snp < 3; ++snp) p->registerOutBigIdx(pos, snp, p->NUM_OUTPUT_FEATURES++);
+      p->NUM_OUTPUT_POSITIONS++;
+      p->registerOutPositionName(pos, name);
+      last_target = pos;
+    }
+  }
+  // tags: all of them up to the 50th one past the last target
+  p->NUM_SAMPLES = 0;
+  p->NUM_INPUT_POSITIONS = 0;
+  p->NUM_INPUT_FEATURES = 0;
+  {
+    std::ifstream tags(challengeFile);
+    uint64_t past_last = 0;
+    while (std::getline(tags, line)) {
+      std::istringstream iss(line);
+      int chrom = 0;
+      uint64_t pos;
+      iss >> chrom;
+      REQUIRE_DRAMATICALLY(chrom == 22, "file format error");
+      iss >> pos;
+      REQUIRE_DRAMATICALLY(iss, "file format error");
+      if (pos > last_target && ++past_last > 50) break;
+      for (uint64_t snp = 0; snp < 3; ++snp) p->registerInBigIdx(pos, snp, p->NUM_INPUT_FEATURES++);
+      p->NUM_INPUT_POSITIONS++;
+      if (p->NUM_SAMPLES == 0) {
+        string pos2, name, g;
+        iss >> pos2 >> name;
+        REQUIRE_DRAMATICALLY(iss, "file format error");
+        while (iss >> g) p->NUM_SAMPLES++;
+      }
+    }
+  }
+  p->NUM_INPUT_FEATURES++;  // the constant
+  REQUIRE_DRAMATICALLY(p->NUM_SAMPLES >= 1 && p->NUM_SAMPLES <= IdashParams::N,
+                       "NUM_SAMPLES must be in [1, N]");
+  p->NUM_REGIONS = IdashParams::N / p->NUM_SAMPLES;
+  p->REGION_SIZE = IdashParams::N / p->NUM_REGIONS;
+  REQUIRE_DRAMATICALLY(p->REGION_SIZE >= p->NUM_SAMPLES, "REGION_SIZE must be >= NUM_SAMPLES ");
+  std::cout << "target_file (headers): " << targetFile << std::endl;
+  std::cout << "tag_file: " << challengeFile << std::endl;
+  std::cout << "NUM_SAMPLES: " << p->NUM_SAMPLES << std::endl;
+  std::cout << "NUM_REGIONS: " << p->NUM_REGIONS << std::endl;
+  std::cout << "REGION_SIZE: " << p->REGION_SIZE << std::endl;
+  std::cout << "NUM_TARGET_POSITIONS: " << p->NUM_OUTPUT_POSITIONS << std::endl;
+  std::cout << "NUM_TAG_POSITIONS: " << p->NUM_INPUT_POSITIONS << std::endl;
+  return p;
+}
+
+IdashKey *keygen_ph2(IdashParams *idashParams) {
+  TLweKey *k = new TLweKey();
+  k->params = IdashParams::tlweParams;
+  std::mt19937_64 rng(fresh_seed());
+  for (uint32_t i = 0; i < IdashParams::N; i += 64) {
+    const uint64_t bits = rng();
+    for (uint32_t j = 0; j < 64; ++j) k->key[i + j] = (int32_t)((bits >> j) & 1u);
+  }
+  return new IdashKey(idashParams, k);
+}
+
+// ---------------------------------------------------------------------------
+// ciphertext files (eval/idash.cpp:513-613): uint64 count, then per record a
+// uint32 index and one sample stream (framing word, a, b, variance).
+
+namespace {
+
+class RecordWriter {
+ public:
+  explicit RecordWriter(const string &filename) : f_(std::fopen(filename.c_str(), "wb")), buf_(kBuf) {}
+  ~RecordWriter() {
+    flush();
+    if (f_) std::fclose(f_);
+  }
+  bool ok() const { return f_ != nullptr; }
+  void raw(const void *p, size_t n) {
+    if (used_ + n > buf_.size()) flush();
+    if (n > buf_.size()) {
+      REQUIRE_DRAMATICALLY(std::fwrite(p, 1, n, f_) == n, "write failed");
+      return;
+    }
+    std::memcpy(buf_.data() + used_, p, n);
+    used_ += n;
+  }
+  void record(uint32_t index, const TLweSample *s) {
+    raw(&index, 4);
+    raw(&kSampleStreamTag, 4);
+    raw(s->a, IdashParams::N * 4);
+    raw(s->b, IdashParams::N * 4);
+    raw(s->current_variance, 8);
+  }
+  void flush() {
+    if (used_) REQUIRE_DRAMATICALLY(std::fwrite(buf_.data(), 1, used_, f_) == used_, "write failed");
+    used_ = 0;
+  }
+
+ private:
+  static constexpr size_t kBuf = 64u << 20;
+  FILE *f_;
+  std::vector<char> buf_;
+  size_t used_ = 0;
+};
+
+// Reads `count` records into a dense block of `slots` samples (slot = index).
+template <typename Map>
+std::shared_ptr<SampleBlock> read_records(const string &filename, size_t slots, Map &map,
+                                          const char *what) {
+  FILE *f = std::fopen(filename.c_str(), "rb");
+  REQUIRE_DRAMATICALLY(f, "Cannot open " << what << " file for read");
+  uint64_t count = 0;
+  REQUIRE_DRAMATICALLY(std::fread(&count, 8, 1, f) == 1, "truncated " << what << " file");
+  auto block = std::make_shared<SampleBlock>(slots);
+  std::vector<char> rec(4 + kSampleStreamBytes);
+  for (uint64_t i = 0; i < count; ++i) {
+    REQUIRE_DRAMATICALLY(std::fread(rec.data(), rec.size(), 1, f) == 1, "truncated " << what << " file");
+    uint32_t index;
+    std::memcpy(&index, rec.data(), 4);
+    REQUIRE_DRAMATICALLY(index < slots, what << " index out of range");
+    TLweSample *s = block->sample(index);
+    REQUIRE_DRAMATICALLY(map.emplace(index, s).second, "duplicate " << what << " index");
+    std::memcpy(s->a, rec.data() + 8, IdashParams::N * 4);
+    std::memcpy(s->b, rec.data() + 8 + IdashParams::N * 4, IdashParams::N * 4);
+    std::memcpy(s->current_variance, rec.data() + 8 + kCtBytes, 8);
+  }
+  std::fclose(f);
+  return block;
+}
+
+size_t num_input_cts(const IdashParams &p) {
+  return (3 * (size_t)p.NUM_INPUT_POSITIONS + p.NUM_REGIONS - 1) / p.NUM_REGIONS;
+}
+
+}  // namespace
+
+void write_encrypted_data(const EncryptedData &encrypted_data, const IdashParams &,
+                          const string &filename) {
+  RecordWriter w(filename);
+  REQUIRE_DRAMATICALLY(w.ok(), "Cannot open encrypted data file for write");
+  const uint64_t n = encrypted_data.enc_data.size();
+  w.raw(&n, 8);
+  for (const auto &e : encrypted_data.enc_data) w.record(e.first, e.second);
+}
+
+void read_encrypted_data(EncryptedData &encrypted_data, const IdashParams &params,
+                         const string &filename) {
+  encrypted_data.block =
+      read_records(filename, num_input_cts(params), encrypted_data.enc_data, "encrypted data");
+}
+
+void write_encrypted_predictions(const EncryptedPredictions &encrypted_preds, const IdashParams &,
+                                 const string &filename) {
+  RecordWriter w(filename);
+  REQUIRE_DRAMATICALLY(w.ok(), "Cannot open encrypted predictions file for write");
+  const uint64_t n = encrypted_preds.score.size();
+  w.raw(&n, 8);
+  for (const auto &e : encrypted_preds.score) w.record(e.first, e.second);
+}
+
+void read_encrypted_predictions(EncryptedPredictions &encrypted_preds, const IdashParams &params,
+                                const string &filename) {
+  encrypted_preds.block = read_records(filename, params.NUM_OUTPUT_FEATURES, encrypted_preds.score,
+                                       "encrypted predictions");
+}
+
+// ---------------------------------------------------------------------------
+// result CSV (eval/idash.cpp:436-469): default ostream formatting of a float
+// is printf's %g with 6 significant digits.
+
+void write_decrypted_predictions(const DecryptedPredictions &predictions, const IdashParams &params,
+                                 const string &filename, const bool PRINT_POS_NAME) {
+  FILE *f = std::fopen(filename.c_str(), "wb");
+  REQUIRE_DRAMATICALLY(f, "Cannot open result file for write");
+  std::vector<char> buf(16u << 20);
+  size_t used = 0;
+  auto emit = [&](const char *s, size_t n) {
+    if (used + n > buf.size()) {
+      std::fwrite(buf.data(), 1, used, f);
+      used = 0;
+    }
+    std::memcpy(buf.data() + used, s, n);
+    used += n;
+  };
+  const char header[] = "Subject ID,target SNP,0,1,2\n";
+  emit(header, sizeof(header) - 1);
+  std::vector<const std::array<std::vector<float>, 3> *> rows;
+  rows.reserve(params.out_position_names.size());
+  for (const auto &e : params.out_position_names) rows.push_back(&predictions.score.at(e.first));
+  char line[256];
+  for (uint64_t s = 0; s < params.NUM_SAMPLES; ++s) {
+    for (size_t t = 0; t < rows.size(); ++t) {
+      const auto &pr = *rows[t];
+      const auto &pn = params.out_position_names[t];
+      const string label = PRINT_POS_NAME ? pn.second : std::to_string(pn.first);
+      const int n = std::snprintf(line, sizeof(line), "%llu,%s,%g,%g,%g\n", (unsigned long long)s,
+                                  label.c_str(), (double)pr[0][s], (double)pr[1][s], (double)pr[2][s]);
+      emit(line, (size_t)n);
+    }
+  }
+  std::fwrite(buf.data(), 1, used, f);
+  std::fclose(f);
+}
+
+// ---------------------------------------------------------------------------
+// encryption (eval/idash.cpp:615-679)
+
+void encrypt_zero_sample(TLweSample *s, const IdashKey &key) {
+  const ihe_layout lay = layout_of(*key.idashParams);
+  const size_t cts = (3 + lay.num_regions - 1) / lay.num_regions;
+  DeviceBuffer d_ct(cts * kCtBytes), d_var(cts * sizeof(double));
+  DeviceKey dk(key.tlweKey);
+  check(ihe_encrypt(engine(), dk.k, &lay, 1, nullptr, fresh_seed(), d_ct.as<int32_t>(),
+                    d_var.as<double>()),
+        "encrypt");
+  check(ihe_memcpy_d2h(engine(), s->a, d_ct.p, IdashParams::N * 4), "copy");
+  check(ihe_memcpy_d2h(engine(), s->b, d_ct.as<int32_t>() + IdashParams::N, IdashParams::N * 4), "copy");
+  check(ihe_memcpy_d2h(engine(), s->current_variance, d_var.p, 8), "copy");
+}
+
+// Phase 1: encryptions of zero, one per FeatIndex of the tag big indices, made
+// on the GPU (a uniform, b = a*s + e) into a dense block.
+void encrypt_data_ph1(EncryptedData &enc_data, const PlaintextData &plain_data, const IdashKey &key) {
+  const IdashParams &p = *key.idashParams;
+  REQUIRE_DRAMATICALLY(plain_data.data.size() == p.NUM_INPUT_POSITIONS, "Incomplete plaintext");
+  for (const auto &e : plain_data.data)
+    REQUIRE_DRAMATICALLY(e.second.size() == p.NUM_SAMPLES, "plaintext dimensions inconsistency");
+  const size_t cts = num_input_cts(p);
+  auto block = std::make_shared<SampleBlock>(cts);
+  if (cts) {
+    const ihe_layout lay = layout_of(p);
+    DeviceBuffer d_ct(cts * kCtBytes), d_var(cts * sizeof(double));
+    DeviceKey dk(key.tlweKey);
+    check(ihe_encrypt(engine(), dk.k, &lay, p.NUM_INPUT_POSITIONS, nullptr, fresh_seed(),
+                      d_ct.as<int32_t>(), d_var.as<double>()),
+          "encrypt");
+    check(ihe_memcpy_d2h(engine(), block->words(), d_ct.p, cts * kCtBytes), "copy");
+    check(ihe_memcpy_d2h(engine(), block->variances(), d_var.p, cts * sizeof(double)), "copy");
+  }
+  // bind every FeatIndex the tags use (eval/idash.cpp:631-637)
+  for (const auto &e : plain_data.data)
+    for (uint64_t snp = 0; snp < 3; ++snp) {
+      const FeatIndex f = p.feature_indexOf(p.inBigIdx(e.first, snp));
+      REQUIRE_DRAMATICALLY(f < cts, "memory error");
+      enc_data.enc_data.emplace(f, block->sample(f));
+    }
+  enc_data.block = block;
+}
+
+// Phase 2: add the one-hot scores (ONE at the genotype's column, the NAN
+// triple when missing) at slot r*RS + sample.
+void encrypt_data_ph2(EncryptedData &enc_data, const PlaintextData &plain_data, const IdashKey &key) {
+  const IdashParams &p = *key.idashParams;
+  REQUIRE_DRAMATICALLY(plain_data.data.size() == p.NUM_INPUT_POSITIONS, "Incomplete plaintext");
+  std::vector<const std::pair<const uint64_t, std::vector<int8_t>> *> rows;
+  for (const auto &e : plain_data.data) rows.push_back(&e);
+#pragma omp parallel for num_threads(NB_THREADS)
+  for (size_t i = 0; i < rows.size(); ++i) {
+    const uint64_t pos = rows[i]->first;
+    const std::vector<int8_t> &g = rows[i]->second;
+    for (uint64_t s = 0; s < p.NUM_SAMPLES; ++s) {
+      const int8_t v = g[s];
+      if (v == 0 || v == 1 || v == 2) {
+        enc_data.setScore(p.inBigIdx(pos, (uint64_t)v), s, IdashParams::ONE_IN_T32, p);
+      } else {
+        enc_data.setScore(p.inBigIdx(pos, 0), s, IdashParams::NAN_0_IN_T32, p);
+        enc_data.setScore(p.inBigIdx(pos, 1), s, IdashParams::NAN_1_IN_T32, p);
+        enc_data.setScore(p.inBigIdx(pos, 2), s, IdashParams::NAN_2_IN_T32, p);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// cloud step (eval/idash.cpp:763-848) on the GPU
+
+void cloud_compute_score(EncryptedPredictions &enc_preds, const EncryptedData &enc_data,
+                         const Model &model, const IdashParams &params) {
+  REQUIRE_DRAMATICALLY(IdashParams::k == 1, "k must be 1");
+  // model -> CSR over output big indices
+  uint32_t num_out = 0;
+  for (const auto &e : model.model) num_out = std::max(num_out, e.first + 1);
+  std::vector<uint32_t> row_ptr(num_out + 1, 0);
+  for (const auto &e : model.model) row_ptr[e.first + 1] = (uint32_t)e.second.size();
+  for (uint32_t o = 0; o < num_out; ++o) row_ptr[o + 1] += row_ptr[o];
+  std::vector<uint32_t> in_bidx(row_ptr[num_out]);
+  std::vector<int32_t> weight(row_ptr[num_out]);
+  for (const auto &e : model.model) {
+    uint32_t at = row_ptr[e.first];
+    for (const auto &c : e.second) {
+      in_bidx[at] = c.first;
+      weight[at++] = c.second;
+    }
+  }
+  // outputs are created once each (eval/idash.cpp:777-779)
+  for (const auto &e : model.model)
+    if (enc_preds.score.count(e.first)) DIE_DRAMATICALLY("output ciphertext created twice");
+
+  const size_t cts = num_input_cts(params);
+  std::shared_ptr<SampleBlock> in = dense_view(enc_data.enc_data, enc_data.block, cts);
+  auto out = std::make_shared<SampleBlock>(num_out);
+  if (num_out) {
+    ihe_ctx *ctx = engine();
+    const ihe_layout lay = layout_of(params);
+    const ihe_model_csr csr = {num_out, row_ptr.data(), in_bidx.data(), weight.data()};
+    ihe_band *band = nullptr;
+    check(ihe_band_create(ctx, &lay, (uint32_t)cts, &csr, IHE_VARIANT_AUTO, &band), "model band");
+    {
+      DeviceBuffer d_in(cts * kCtBytes), d_in_var(cts * sizeof(double));
+      DeviceBuffer d_out(num_out * kCtBytes), d_out_var(num_out * sizeof(double));
+      if (cts) {
+        check(ihe_memcpy_h2d(ctx, d_in.p, in->words(), cts * kCtBytes), "copy");
+        check(ihe_memcpy_h2d(ctx, d_in_var.p, in->variances(), cts * sizeof(double)), "copy");
+      }
+      check(ihe_cloud_score(ctx, band, d_in.as<int32_t>(), d_in_var.as<double>(), d_out.as<int32_t>(),
+                            d_out_var.as<double>()),
+            "cloud");
+      check(ihe_memcpy_d2h(ctx, out->words(), d_out.p, num_out * kCtBytes), "copy");
+      check(ihe_memcpy_d2h(ctx, out->variances(), d_out_var.p, num_out * sizeof(double)), "copy");
+    }
+    ihe_band_destroy(band);
+  }
+  for (const auto &e : model.model) enc_preds.score.emplace(e.first, out->sample(e.first));
+  enc_preds.block = out;
+}
+
+// ---------------------------------------------------------------------------
+// decryption (eval/idash.cpp:681-761): float(t32tod(phase_j)), j < NUM_SAMPLES
+
+void decrypt_predictions(DecryptedPredictions &predictions, const EncryptedPredictions &enc_preds,
+                         const IdashKey &key) {
+  const IdashParams &p = *key.idashParams;
+  const uint32_t ns = p.NUM_SAMPLES;
+  for (const auto &e : p.out_features_index)
+    for (int snp = 0; snp < 3; ++snp) predictions.score[e.first][snp].resize(ns);
+  uint32_t num = 0;
+  for (const auto &e : enc_preds.score) num = std::max(num, e.first + 1);
+  for (const auto &e : p.out_features_index)
+    for (FeatBigIndex b : e.second) {
+      REQUIRE_DRAMATICALLY(enc_preds.score.count(b), "missing prediction ciphertext");
+      num = std::max(num, b + 1);
+    }
+  if (num == 0) return;
+  std::shared_ptr<SampleBlock> in = dense_view(enc_preds.score, enc_preds.block, num);
+  std::vector<float> plain((size_t)num * ns);
+  {
+    ihe_ctx *ctx = engine();
+    DeviceBuffer d_in(num * kCtBytes), d_out(plain.size() * sizeof(float));
+    DeviceKey dk(key.tlweKey);
+    check(ihe_memcpy_h2d(ctx, d_in.p, in->words(), num * kCtBytes), "copy");
+    check(ihe_decrypt(ctx, dk.k, ns, d_in.as<int32_t>(), num, d_out.as<float>()), "decrypt");
+    check(ihe_memcpy_d2h(ctx, plain.data(), d_out.p, plain.size() * sizeof(float)), "copy");
+  }
+  for (const auto &e : p.out_features_index)
+    for (int snp = 0; snp < 3; ++snp)
+      std::memcpy(predictions.score[e.first][snp].data(), plain.data() + (size_t)e.second[snp] * ns,
+                  ns * sizeof(float));
+}
+
+// ---------------------------------------------------------------------------
+// plaintext model (eval/idash.cpp:850-927), on the CPU: the validation path
+
+PlaintextOnehot compute_plaintext_onehot(const PlaintextData &X, const IdashParams &params) {
+  PlaintextOnehot oh;
+  for (const auto &e : params.in_features_index) {
+    const uint64_t pos = e.first;
+    for (int v = 0; v < 3; ++v) oh[e.second[v]].resize(params.NUM_SAMPLES);
+    const std::vector<int8_t> &g = X.data.at(pos);
+    for (uint64_t s = 0; s < params.NUM_SAMPLES; ++s) {
+      double val[3];
+      switch (g[s]) {
+        case 0: val[0] = params.ONE_IN_D, val[1] = 0, val[2] = 0; break;
+        case 1: val[0] = 0, val[1] = params.ONE_IN_D, val[2] = 0; break;
+        case 2: val[0] = 0, val[1] = 0, val[2] = params.ONE_IN_D; break;
+        default: val[0] = params.NAN_0_IN_D, val[1] = params.NAN_1_IN_D, val[2] = params.NAN_2_IN_D;
+      }
+      for (int v = 0; v < 3; ++v) oh[e.second[v]][s] = val[v];
+    }
+  }
+  return oh;
+}
+
+void compute_score(DecryptedPredictions &predictions, const PlaintextData &X, const Model &M,
+                   const IdashParams &params) {
+  PlaintextOnehot oh = compute_plaintext_onehot(X, params);
+  for (const auto &e : params.out_features_index) {
+    for (int snp = 0; snp < 3; ++snp) {
+      std::vector<float> &out = predictions.score[e.first][snp];
+      out.assign(params.NUM_SAMPLES, 0.0f);
+      for (const auto &c : M.model.at(e.second[snp])) {
+        for (uint64_t s = 0; s < params.NUM_SAMPLES; ++s) {
+          if (c.first == params.constant_bigIndex())
+            out[s] += c.second * IdashParams::ONE_IN_D;
+          else
+            out[s] += c.second * oh[c.first][s];
+        }
+      }
+    }
+  }
+}
+
+bool DecryptedPredictions::testEquals(const DecryptedPredictions &val1,
+                                      const DecryptedPredictions &val2, const IdashParams &params) {
+  for (uint64_t s = 0; s < params.NUM_SAMPLES; ++s)
+    for (const auto &e : params.out_features_index)
+      for (int snp = 0; snp < 3; ++snp) {
+        const float x = val1.score.at(e.first)[snp][s], y = val2.score.at(e.first)[snp][s];
+        if (x != y) std::cout << x << " " << y << std::endl;
+      }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// profiler (eval/idash.cpp:929-967)
+
+Profiler::Profiler() : tw0(universalWallTime()), tc0(universalClockTime()) {}
+
+double Profiler::universalWallTime() {
+  struct timeval tv;
+  if (gettimeofday(&tv, nullptr)) return 0;
+  return (double)tv.tv_sec + (double)tv.tv_usec * 1e-6;
+}
+
+double Profiler::universalClockTime() { return (double)clock() / CLOCKS_PER_SEC; }
+double Profiler::walltime() const { return universalWallTime() - tw0; }
+double Profiler::clocktime() const { return universalClockTime() - tc0; }
+
+long int Profiler::maxrss() const {
+  struct rusage u;
+  if (getrusage(RUSAGE_SELF, &u)) return -1;
+  return u.ru_maxrss * 1000;
+}

@@ -184,3 +184,41 @@ def cloud_bytes(c: Cohort) -> int:
     start per target."""
     return (8192 * (c.num_in_cts + c.num_outputs) + 2 * 3 * c.neighbours * c.num_outputs
             + 4 * c.num_outputs + 4 * c.num_targets)
+
+
+def write_reference_files(c: Cohort, directory: str, nan_fraction: float = 0.0,
+                          seed: int = 0) -> dict:
+    """Writes the cohort in the reference's input formats:
+      tags.txt     "22 pos pos+1 name g_0 .. g_{NS-1}" per tag, "NA" when missing
+                   (read by keygen_ph1 and read_plaintext_data, eval/idash.cpp:252-311, 374-405)
+      targets.txt  one target position per line (the positions-only target file,
+                   eval/idash.cpp:327-343)
+      model/       "<pos>_<k>.hr" per target and class with "<tagpos>_<v> <w>" lines and
+                   a "Constant <bias>" line (eval/parse_vw.cpp:7-29, eval/idash.cpp:66-90)
+    Returns the paths and the genotypes written (with -1 where missing)."""
+    import os
+    os.makedirs(os.path.join(directory, "model"), exist_ok=True)
+    geno = c.geno.copy()
+    if nan_fraction > 0:
+        rng = np.random.default_rng(seed)
+        geno[rng.random(geno.shape) < nan_fraction] = -1
+    tags = os.path.join(directory, "tags.txt")
+    with open(tags, "w") as f:
+        for q, pos in enumerate(c.tag_pos.tolist()):
+            g = " ".join("NA" if v < 0 else str(int(v)) for v in geno[q])
+            f.write(f"22 {pos} {pos + 1} rs{pos} {g}\n")
+    targets = os.path.join(directory, "targets.txt")
+    with open(targets, "w") as f:
+        f.write("".join(f"{p}\n" for p in c.target_pos.tolist()))
+    n = c.neighbours
+    tag_pos = c.tag_pos.tolist()
+    for t, pos in enumerate(c.target_pos.tolist()):
+        lo = int(c.lo[t])
+        for k in range(3):
+            o = 3 * t + k
+            lines = [f"{tag_pos[lo + j // 3]}_{j % 3} {int(c.weights[o, j])}\n" for j in range(3 * n)]
+            lines.append(f"Constant {int(c.bias[o])}\n")
+            with open(os.path.join(directory, "model", f"{pos}_{k}.hr"), "w") as f:
+                f.write("".join(lines))
+    return {"tags": tags, "targets": targets, "model": os.path.join(directory, "model"),
+            "geno": geno}

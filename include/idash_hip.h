@@ -147,6 +147,8 @@ int ihe_key_destroy(ihe_key *key);
 /* Replaces encrypt_data_ph1 + encrypt_data_ph2 (eval/idash.cpp:615-679;
  * eval/idash.h:248-249). d_geno: [num_tags][NS] int8 genotypes in tag order
  * (tag q owns big indices 3q..3q+2); values outside {0,1,2} are missing.
+ * d_geno == NULL encrypts zero in every slot: encrypt_data_ph1 alone
+ * (eval/idash.cpp:615-640), whose scores the caller adds afterwards.
  * Writes ceil(3*num_tags / R) ciphertexts (FeatIndex order) to d_out and
  * alpha^2 = 2^-50 to d_out_var. seed selects the counter-based random streams. */
 int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint32_t num_tags,

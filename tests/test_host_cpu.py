@@ -1,0 +1,98 @@
+"""The idash-compatible C++ host on the CPU: keygen (eval/idash.cpp:315-434)
+and the parameter/key files it writes (eval/idash.cpp:128-246, 476-502),
+checked against the index rules of the reference; error behaviour
+(REQUIRE_DRAMATICALLY: "ERROR: ..." on stdout, then abort)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from idash2019_2_amd import formats, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "idash2019_2_amd", "host")
+
+
+def run(exe, *args, cwd, env=None):
+    path = os.path.join(HOST, exe)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is not built (run __graft_entry__.build())")
+    e = dict(os.environ, IHE_SEED="5", **(env or {}))
+    return subprocess.run([path, *args], cwd=cwd, env=e, capture_output=True, text=True, timeout=120)
+
+
+def test_keygen_writes_reference_layout(tmp_path):
+    c = synth.generate(60, 200, 128, 5, seed=11)
+    synth.write_reference_files(c, str(tmp_path))
+    r = run("keygen", "targets.txt", "tags.txt", "1", cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for line in ("NUM_SAMPLES: 128", "NUM_REGIONS: 8", "REGION_SIZE: 128",
+                 "NUM_TARGET_POSITIONS: 200", "NUM_TAG_POSITIONS: 60"):
+        assert line in r.stdout
+    p = formats.read_params(str(tmp_path / "params.bin"))
+    assert (p["NUM_SAMPLES"], p["NUM_REGIONS"], p["REGION_SIZE"]) == (128, 8, 128)
+    assert p["NUM_INPUT_POSITIONS"] == 60 and p["NUM_OUTPUT_POSITIONS"] == 200
+    assert p["NUM_INPUT_FEATURES"] == 3 * 60 + 1          # + the constant
+    assert p["NUM_OUTPUT_FEATURES"] == 600
+    # tag q (file order) owns big indices 3q..3q+2; target t owns 3t..3t+2
+    for q, pos in enumerate(c.tag_pos.tolist()):
+        assert p["in_features_index"][pos] == (3 * q, 3 * q + 1, 3 * q + 2)
+    assert [pos for pos, _ in p["out_position_names"]] == c.target_pos.tolist()
+    assert all(name == str(pos) for pos, name in p["out_position_names"])
+    for t, pos in enumerate(c.target_pos.tolist()):
+        assert p["out_features_index"][pos] == (3 * t, 3 * t + 1, 3 * t + 2)
+    pk, key, tp = formats.read_key(str(tmp_path / "keys.bin"))
+    assert pk["NUM_SAMPLES"] == 128 and pk["in_features_index"] == p["in_features_index"]
+    assert tp == dict(N=1024, k=1, alpha_min=2.0 ** -25, alpha_max=0.25)
+    assert set(np.unique(key)) <= {0, 1} and 400 < key.sum() < 624
+
+
+def test_keygen_layout_for_odd_sample_counts(tmp_path):
+    c = synth.generate(20, 30, 100, 3, seed=2)        # R = 10, RS = 102
+    synth.write_reference_files(c, str(tmp_path))
+    assert run("keygen", "targets.txt", "tags.txt", "1", cwd=tmp_path).returncode == 0
+    p = formats.read_params(str(tmp_path / "params.bin"))
+    assert (p["NUM_SAMPLES"], p["NUM_REGIONS"], p["REGION_SIZE"]) == (100, 10, 102)
+
+
+def test_keygen_keeps_fifty_tags_past_the_last_target(tmp_path):
+    # 30 tags, 5 targets, then 70 more tags: keygen reads until the 51st tag past
+    # the last target (eval/idash.cpp:383-386)
+    tags = [1000 + 10 * i for i in range(100)]
+    targets = [1003 + 10 * i for i in range(0, 30, 6)]
+    with open(tmp_path / "tags.txt", "w") as f:
+        for p in tags:
+            f.write(f"22 {p} {p + 1} rs{p} 0 1 2 NA\n")
+    with open(tmp_path / "targets.txt", "w") as f:
+        f.write("".join(f"{t}\n" for t in targets))
+    assert run("keygen", "targets.txt", "tags.txt", "1", cwd=tmp_path).returncode == 0
+    p = formats.read_params(str(tmp_path / "params.bin"))
+    past = [t for t in tags if t > targets[-1]]
+    assert p["NUM_INPUT_POSITIONS"] == len(tags) - len(past) + 50
+    assert p["NUM_SAMPLES"] == 4 and p["NUM_REGIONS"] == 256 and p["REGION_SIZE"] == 4
+
+
+def test_keygen_full_target_format_keeps_names(tmp_path):
+    c = synth.generate(15, 10, 64, 3, seed=4)
+    synth.write_reference_files(c, str(tmp_path))
+    with open(tmp_path / "targets_full.txt", "w") as f:
+        for pos in c.target_pos.tolist():
+            f.write(f"22 {pos} {pos + 1} snp{pos} 0 0\n")
+    assert run("keygen", "targets_full.txt", "tags.txt", "0", cwd=tmp_path).returncode == 0
+    p = formats.read_params(str(tmp_path / "params.bin"))
+    assert [n for _, n in p["out_position_names"]] == [f"snp{x}" for x in c.target_pos.tolist()]
+
+
+def test_bad_tag_file_dies_dramatically(tmp_path):
+    (tmp_path / "targets.txt").write_text("100\n")
+    (tmp_path / "tags.txt").write_text("21 90 91 rs90 0 1\n")
+    r = run("keygen", "targets.txt", "tags.txt", "1", cwd=tmp_path)
+    assert r.returncode != 0 and "ERROR: file format error" in r.stdout
+
+
+def test_missing_files_die_dramatically(tmp_path):
+    r = run("cloud", str(tmp_path / "model"), cwd=tmp_path)
+    assert r.returncode != 0 and "ERROR: Cannot open parameters file for read" in r.stdout
+    r = run("decrypt", cwd=tmp_path)
+    assert r.returncode != 0 and "ERROR: Cannot open key file for read" in r.stdout
