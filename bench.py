@@ -144,6 +144,20 @@ def main(argv=None):
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": read_traffic(args.workload, variant_name)}
 
+        # encrypt and decrypt kernels on the same workload (outside the timed steps)
+        eng.timing(True)
+        eng.timing_reset()
+        d_dec = eng.empty((c.num_outputs, ns), np.float32)
+        for _ in range(3):
+            eng.encrypt(key, ns, d_geno, 0x5EED, d_ct, d_var)
+            eng.decrypt(key, ns, d_out, c.num_outputs, d_dec)
+        enc_ms, enc_n = eng.timing_read(_abi.TIMER_ENCRYPT)
+        dec_ms, dec_n = eng.timing_read(_abi.TIMER_DECRYPT)
+        eng.timing(False)
+        d_dec.free()
+        log(f"encrypt kernel avg {enc_ms / enc_n:.4f} ms ({C} ciphertexts), decrypt kernel avg "
+            f"{dec_ms / dec_n:.4f} ms ({c.num_outputs} ciphertexts)")
+
         cpu = None
         if not args.no_cpu_baseline:
             threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) \
@@ -171,7 +185,9 @@ def main(argv=None):
                    "variant": variant_name, "algorithmic_bytes": alg_bytes,
                    "target_positions_per_s": round(c.num_targets / step_s, 1),
                    "cloud_kernel_ms": round(avg_kernel_s * 1e3, 4),
-                   "variance_kernel_ms": round(var_ms / kern_n, 4)},
+                   "variance_kernel_ms": round(var_ms / kern_n, 4),
+                   "encrypt_kernel_ms": round(enc_ms / enc_n, 4),
+                   "decrypt_kernel_ms": round(dec_ms / dec_n, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
