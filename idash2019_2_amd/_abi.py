@@ -40,6 +40,21 @@ class ModelCSR(ctypes.Structure):
     _fields_ = [("num_outputs", _u32), ("row_ptr", _vp), ("in_bidx", _vp), ("weight", _vp)]
 
 
+class Cohort(ctypes.Structure):
+    _fields_ = [("band", _vp), ("layout", Layout), ("num_in_cts", _u32), ("d_in", _vp),
+                ("d_in_var", _vp), ("d_out", _vp), ("d_out_var", _vp)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("num_cohorts", _u32), ("num_launches", _u32), ("num_tiles", _u32),
+                ("captured", _u32)]
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("rows_per_tile", _u32), ("kblocks", _u32), ("col_tiles_per_wave", _u32),
+                ("ms", ctypes.c_float)]
+
+
 class BandInfo(ctypes.Structure):
     _fields_ = [("num_outputs", _u32), ("num_groups", _u32), ("num_tiles", _u32),
                 ("num_cols", _u32), ("nnz", _u32), ("variant", _u32), ("max_window", _u32),
@@ -75,6 +90,15 @@ SIGNATURES = {
     "ihe_band_plan": (_int, [ctypes.POINTER(Layout), _u32, ctypes.POINTER(ModelCSR), _int,
                              ctypes.POINTER(BandInfo)]),
     "ihe_cloud_score": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "ihe_cloud_score_layout": (_int, [_vp, _vp, ctypes.POINTER(Layout), _u32, _vp, _vp, _vp, _vp]),
+    "ihe_band_create_tuned": (_int, [_vp, ctypes.POINTER(Layout), _u32, ctypes.POINTER(ModelCSR),
+                                     _vp, _vp, _u32, _pvp, ctypes.POINTER(TuneEntry),
+                                     ctypes.POINTER(_u32)]),
+    "ihe_batch_create": (_int, [_vp, ctypes.POINTER(Cohort), _u32, _pvp]),
+    "ihe_batch_capture": (_int, [_vp]),
+    "ihe_batch_run": (_int, [_vp]),
+    "ihe_batch_get_info": (_int, [_vp, ctypes.POINTER(BatchInfo)]),
+    "ihe_batch_destroy": (_int, [_vp]),
     "ihe_key_create": (_int, [_vp, _vp, _pvp]),
     "ihe_key_destroy": (_int, [_vp]),
     "ihe_encrypt": (_int, [_vp, _vp, ctypes.POINTER(Layout), _u32, _vp, _u64, _vp, _vp]),

@@ -1,0 +1,230 @@
+// Batched cloud step: many cohorts (each with its own sample count, input
+// ciphertexts and resident model band) in one launch per MFMA kernel family
+// plus one variance launch, optionally captured into a hipGraph so that a
+// step is a single graph launch.
+//
+// The reference runs one cloud_compute_score per cohort (eval/cloud.cpp:17);
+// here the per-cohort work lists are concatenated. A launch covers every
+// cohort whose band uses the same kernel family; its workgroups find their
+// (cohort segment, tile) by a binary search over the family's entry table,
+// and consecutive workgroups of an XCD walk consecutive tiles of one cohort.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ihe_band.h"
+
+using namespace ihe;
+
+struct ihe_batch {
+  ihe_ctx *ctx = nullptr;
+  struct Launch {
+    MfmaShape shape;
+    bool rot;
+    uint32_t entry0, num_entries, first0, num_items;
+  };
+  std::vector<Launch> launches;
+  uint32_t num_cohorts = 0, num_items = 0;
+  uint32_t num_ventries = 0, num_vblocks = 0;
+  void *dmem = nullptr;
+  MfmaEntry *d_entries = nullptr;
+  uint32_t *d_first = nullptr;
+  VarEntry *d_ventries = nullptr;
+  uint32_t *d_vfirst = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+struct Keyed {
+  MfmaShape shape;
+  bool rot;
+  uint32_t cohort;
+  MfmaEntry e;
+  uint32_t tiles;
+};
+
+bool same_family(const Keyed &a, const Keyed &b) {
+  return a.shape.rb == b.shape.rb && a.shape.nkb == b.shape.nkb && a.shape.ntw == b.shape.ntw &&
+         a.rot == b.rot;
+}
+
+void enqueue(const ihe_batch *b, hipStream_t s) {
+  MfmaEntry none = {};
+  for (const ihe_batch::Launch &L : b->launches)
+    launch_mfma(s, L.shape, L.rot, none, b->d_entries + L.entry0, b->d_first + L.first0,
+                L.num_entries, L.num_items);
+  if (b->num_ventries) {
+    VarEntry vnone = {};
+    launch_var(s, vnone, b->d_ventries, b->d_vfirst, b->num_ventries, b->num_vblocks);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohorts,
+                     ihe_batch **out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!out || (num_cohorts && !cohorts)) return fail(IHE_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::vector<Keyed> keyed;
+  std::vector<VarEntry> vent;
+  std::vector<uint32_t> vfirst(1, 0);
+  for (uint32_t i = 0; i < num_cohorts; ++i) {
+    const ihe_cohort &c = cohorts[i];
+    const std::string which = "cohort " + std::to_string(i) + ": ";
+    const ihe_band *band = c.band;
+    if (!band || band->ctx != ctx) return fail(IHE_ERR_ARG, which + "band does not belong to this context");
+    if (band->mode != kModeMfma)
+      return fail(IHE_ERR_UNSUPPORTED, which + "batches run MFMA bands only");
+    const ihe_layout &L = c.layout;
+    if (L.num_samples == 0 || L.num_samples > kN || L.num_regions != kN / L.num_samples ||
+        L.region_size != kN / L.num_regions)
+      return fail(IHE_ERR_ARG, which + "layout must satisfy R = N / NS and RS = N / R");
+    if (band->has_cols && band->max_bidx / L.num_regions >= c.num_in_cts)
+      return fail(IHE_ERR_RANGE, which + "the model references FeatIndex " +
+                                     std::to_string(band->max_bidx / L.num_regions) + " >= " +
+                                     std::to_string(c.num_in_cts));
+    if (band->num_out == 0) continue;
+    if (!c.d_out || (c.num_in_cts && !c.d_in)) return fail(IHE_ERR_ARG, which + "null ciphertext buffer");
+    if (c.d_out_var && c.num_in_cts && !c.d_in_var) return fail(IHE_ERR_ARG, which + "null input variance");
+    for (const Segment &s : band->segments) {
+      Keyed k;
+      k.shape = band->shape;
+      k.rot = L.num_regions > 1;
+      k.cohort = i;
+      k.tiles = s.tile_end - s.tile_begin;
+      MfmaEntry &E = k.e;
+      E = {};
+      E.tiles = band->d_mtiles;
+      E.kcols = band->d_kcols;
+      E.adata = band->d_adata;
+      E.rowc = band->d_rowc;
+      E.ns = L.num_samples;
+      E.rs = L.region_size;
+      E.R = L.num_regions;
+      E.ct_base = s.in_base / L.num_regions;
+      const uint32_t in_cts = c.num_in_cts > E.ct_base ? std::min(c.num_in_cts - E.ct_base, kSegCts) : 0u;
+      E.in = in_cts ? c.d_in + (size_t)E.ct_base * kCtWords : nullptr;
+      E.in_bytes = in_cts * kCtWords * 4u;
+      E.out = c.d_out + (size_t)s.out_base * kCtWords;
+      E.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
+      E.tile_base = s.tile_begin;
+      if (k.tiles) keyed.push_back(k);
+    }
+    if (c.d_out_var) {
+      vent.push_back({band->d_var_ptr, band->d_var_col, band->d_var_w2, c.d_in_var, c.d_out_var,
+                      band->num_out, L.num_regions, {0, 0}});
+      vfirst.push_back(vfirst.back() + (band->num_out + 255) / 256);
+    }
+  }
+  // one launch per kernel family, cohorts in order within it
+  std::stable_sort(keyed.begin(), keyed.end(), [](const Keyed &a, const Keyed &b) {
+    if (a.rot != b.rot) return a.rot < b.rot;
+    if (a.shape.nkb != b.shape.nkb) return a.shape.nkb < b.shape.nkb;
+    if (a.shape.rb != b.shape.rb) return a.shape.rb < b.shape.rb;
+    return a.shape.ntw < b.shape.ntw;
+  });
+  ihe_batch *b = new ihe_batch();
+  b->ctx = ctx;
+  b->num_cohorts = num_cohorts;
+  std::vector<MfmaEntry> entries;
+  std::vector<uint32_t> first;
+  for (size_t i = 0; i < keyed.size();) {
+    size_t j = i;
+    ihe_batch::Launch L{keyed[i].shape, keyed[i].rot, (uint32_t)entries.size(), 0,
+                        (uint32_t)first.size(), 0};
+    uint32_t items = 0;
+    for (; j < keyed.size() && same_family(keyed[i], keyed[j]); ++j) {
+      entries.push_back(keyed[j].e);
+      first.push_back(items);
+      items += keyed[j].tiles;
+    }
+    first.push_back(items);
+    L.num_entries = (uint32_t)(j - i);
+    L.num_items = items;
+    b->num_items += items;
+    b->launches.push_back(L);
+    i = j;
+  }
+  b->num_ventries = (uint32_t)vent.size();
+  b->num_vblocks = vfirst.back();
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_first = align(entries.size() * sizeof(MfmaEntry));
+  const size_t o_vent = o_first + align(first.size() * 4);
+  const size_t o_vfirst = o_vent + align(vent.size() * sizeof(VarEntry));
+  const size_t bytes = o_vfirst + align(vfirst.size() * 4);
+  if (int rc = ihe_malloc(ctx, bytes, &b->dmem)) {
+    delete b;
+    return rc;
+  }
+  std::vector<char> h(bytes, 0);
+  if (!entries.empty()) std::memcpy(h.data(), entries.data(), entries.size() * sizeof(MfmaEntry));
+  if (!first.empty()) std::memcpy(h.data() + o_first, first.data(), first.size() * 4);
+  if (!vent.empty()) std::memcpy(h.data() + o_vent, vent.data(), vent.size() * sizeof(VarEntry));
+  std::memcpy(h.data() + o_vfirst, vfirst.data(), vfirst.size() * 4);
+  char *d = static_cast<char *>(b->dmem);
+  b->d_entries = reinterpret_cast<MfmaEntry *>(d);
+  b->d_first = reinterpret_cast<uint32_t *>(d + o_first);
+  b->d_ventries = reinterpret_cast<VarEntry *>(d + o_vent);
+  b->d_vfirst = reinterpret_cast<uint32_t *>(d + o_vfirst);
+  if (int rc = ihe_memcpy_h2d(ctx, b->dmem, h.data(), bytes)) {
+    (void)ihe_free(ctx, b->dmem);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return IHE_OK;
+}
+
+int ihe_batch_run(ihe_batch *b) {
+  if (!b) return fail(IHE_ERR_ARG, "null batch");
+  if (int rc = make_current(b->ctx)) return rc;
+  ScopedTimer t(b->ctx, IHE_TIMER_BATCH);
+  if (b->exec) {
+    IHE_HIP(hipGraphLaunch(b->exec, b->ctx->stream));
+  } else {
+    enqueue(b, b->ctx->stream);
+    IHE_HIP(hipGetLastError());
+  }
+  return IHE_OK;
+}
+
+int ihe_batch_capture(ihe_batch *b) {
+  if (!b) return fail(IHE_ERR_ARG, "null batch");
+  if (int rc = make_current(b->ctx)) return rc;
+  if (b->exec) return IHE_OK;
+  IHE_HIP(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
+  enqueue(b, b->ctx->stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->ctx->stream, &g);
+  if (e != hipSuccess) return fail(IHE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  b->graph = g;
+  IHE_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return IHE_OK;
+}
+
+int ihe_batch_get_info(const ihe_batch *b, ihe_batch_info *info) {
+  if (!b || !info) return fail(IHE_ERR_ARG, "null argument");
+  info->num_cohorts = b->num_cohorts;
+  info->num_launches = (uint32_t)b->launches.size() + (b->num_ventries ? 1u : 0u);
+  info->num_tiles = b->num_items;
+  info->captured = b->exec ? 1u : 0u;
+  return IHE_OK;
+}
+
+int ihe_batch_destroy(ihe_batch *b) {
+  if (!b) return IHE_OK;
+  (void)ihe_sync(b->ctx);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)ihe_free(b->ctx, b->dmem);
+  delete b;
+  return IHE_OK;
+}
+
+}  // extern "C"

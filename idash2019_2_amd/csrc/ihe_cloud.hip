@@ -37,6 +37,7 @@
 #include <unordered_map>
 
 #include "ihe_internal.h"
+#include "ihe_band.h"
 
 namespace ihe {
 
@@ -44,7 +45,6 @@ constexpr int kOmax = 12;                    // outputs per group (register-bloc
 constexpr int kOtab = 16;                    // output-table entries per group
 constexpr int kDot2ChunkCols = 16;           // 8 column pairs per chunk (dot2)
 constexpr int kWideChunkCols = 8;            // 8 columns per chunk (32-bit weights)
-constexpr uint32_t kSegCts = (1u << 18) - 1; // ciphertexts per segment: byte offsets stay below 2^31
 constexpr int kRsrcWord3 = 0x00020000;       // gfx9 raw buffer descriptor, dword 3
 
 #define IHE_CONSTANT __attribute__((address_space(4)))
@@ -291,22 +291,6 @@ __global__ __launch_bounds__(256) void k_cloud(CloudParams P) {
   }
 }
 
-// var_o = sum over region-0 entries, in the model's visiting order, of
-// (double)(int32)(w*w) * var_in (tLweAddMulTo's variance update; the rotated
-// regions never carry a variance into the output, eval/idash.cpp:786, :835).
-__global__ __launch_bounds__(256) void k_cloud_var(const uint32_t *__restrict__ var_ptr,
-                                                   const uint32_t *__restrict__ var_ct,
-                                                   const int32_t *__restrict__ var_w2,
-                                                   const double *__restrict__ var_in,
-                                                   double *__restrict__ var_out, uint32_t num_out) {
-  const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= num_out) return;
-  double v = 0.0;
-  for (uint32_t e = var_ptr[o]; e < var_ptr[o + 1]; ++e)
-    v = __dadd_rn(v, __dmul_rn((double)var_w2[e], var_in[var_ct[e]]));
-  var_out[o] = v;
-}
-
 // ---------------------------------------------------------------------------
 // LDS-staged kernel for narrow windows (<= 32 columns per group, int16
 // weights). A workgroup owns one tile (consecutive groups) and one 256-word
@@ -453,571 +437,10 @@ __global__ __launch_bounds__(256) void k_cloud_staged(StagedParams P) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// Byte-split int8 MFMA kernel. A tile is 32 consecutive output rows over the
-// union of their columns (K padded to blocks of 32). Ciphertext words are
-// recoded into balanced signed bytes, x = sum_b 2^(8b) d_b with
-// d_b = byte_b(x + 0x80808080) - 128 in [-128, 127] (exact mod 2^32), and
-// weights into w = w_0 + 2^8 w_1 with w_0 = the sign-extended low byte, so
-//   sum_k w x = sum_{b+h<=3} 2^(8(b+h)) sum_k w_h d_b    (mod 2^32)
-// since terms with b+h >= 4 vanish. Each K-block of 32 columns costs seven
-// v_mfma_i32_32x32x32_i8 into four accumulators (shifts 0, 8, 16, 24), then
-// the accumulators are recombined with shifts; every partial sum stays below
-// 2^31 in magnitude.
-//
-// Operand layout (v_mfma_i32_32x32x32_i8): A lane l holds row l%32, bytes
-// k = 16*(l/32) + j; B lane l holds column l%32 with the same k; result value
-// v of lane l is row 8*(v/4) + 4*(l/32) + v%4, column l%32.
-typedef int int4v __attribute__((ext_vector_type(4)));
-typedef int int16v __attribute__((ext_vector_type(16)));
-constexpr uint32_t kMfmaRows = 32;  // outputs per tile
-constexpr uint32_t kMfmaMaxKb = 16; // K-blocks (of 32 columns) per tile
-
-struct MTileDev {
-  uint32_t kcol_off;  // first column word of the tile (32 per K-block)
-  uint32_t nkb;       // K-blocks
-  uint32_t a_off;     // first A operand block (64 lanes x 16 bytes; 2 per K-block)
-  uint32_t row_off;   // 32 row biases
-  uint32_t slot0;     // first output slot of the tile, relative to the segment
-  uint32_t nrows;     // rows in use (<= 32)
-  uint32_t pad0, pad1;
-};
-
-struct MfmaParams {
-  const MTileDev *tiles;
-  const uint32_t *kcols;
-  const int4v *adata;
-  const uint32_t *rowc;
-  const int32_t *in;
-  int32_t *out;
-  uint32_t in_bytes, out_bytes;
-  uint32_t ns, rs, tile_base;
-};
-
-// 16 words -> 4 planes of balanced signed bytes (plane b = digit b of each word).
-__device__ __forceinline__ void byte_planes(const uint32_t (&x)[16], int4v (&p)[4]) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    uint32_t y[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) y[i] = (x[4 * q + i] + 0x80808080u) ^ 0x80808080u;
-    const uint32_t t0 = __builtin_amdgcn_perm(y[1], y[0], 0x05010400u);
-    const uint32_t t1 = __builtin_amdgcn_perm(y[1], y[0], 0x07030602u);
-    const uint32_t t2 = __builtin_amdgcn_perm(y[3], y[2], 0x05010400u);
-    const uint32_t t3 = __builtin_amdgcn_perm(y[3], y[2], 0x07030602u);
-    p[0][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x05040100u);
-    p[1][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x07060302u);
-    p[2][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x05040100u);
-    p[3][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x07060302u);
-  }
-}
-
-// Loads the 16 column words this lane contributes to K-block kb at `word`.
-template <bool ROT>
-__device__ __forceinline__ void mfma_load(__amdgpu_buffer_rsrc_t in_r,
-                                          const IHE_CONSTANT uint32_t *kc, uint32_t kb, uint32_t h,
-                                          uint32_t word, uint32_t (&x)[16]) {
-  const u32x16 cl = *(const IHE_CONSTANT u32x16 *)(kc + 32 * kb);
-  const u32x16 chh = *(const IHE_CONSTANT u32x16 *)(kc + 32 * kb + 16);
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const uint32_t cw = h ? chh[j] : cl[j];
-    if constexpr (!ROT) {
-      x[j] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(in_r, cw + word * 4u, 0, 0);
-    } else {
-      const uint32_t t = (word & (kN - 1)) + ((cw >> 2) & (kN - 1));
-      const uint32_t v = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
-          in_r, (cw & ~0x1FFFu) + ((t & (kN - 1)) << 2) + (word & kN) * 4u, 0, 0);
-      x[j] = t >= kN ? 0u - v : v;
-    }
-  }
-}
-
-template <bool ROT, int NTW>
-__global__ __launch_bounds__(256) void k_cloud_mfma(MfmaParams P) {
-  constexpr uint32_t wg_per_tile = kCtWords / (4 * NTW * 32);
-  const uint32_t tile = P.tile_base + blockIdx.x / wg_per_tile;
-  const uint32_t part = blockIdx.x % wg_per_tile;
-  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const uint32_t h = lane / 32, r = lane % 32;
-  const MTileDev T = __builtin_bit_cast(MTileDev, *(const IHE_CONSTANT u32x8 *)(P.tiles + tile));
-  const __amdgpu_buffer_rsrc_t in_r =
-      __builtin_amdgcn_make_buffer_rsrc((void *)P.in, (short)0, (int)P.in_bytes, kRsrcWord3);
-  const __amdgpu_buffer_rsrc_t out_r =
-      __builtin_amdgcn_make_buffer_rsrc((void *)P.out, (short)0, (int)P.out_bytes, kRsrcWord3);
-  const IHE_CONSTANT uint32_t *kc = cptr(P.kcols) + T.kcol_off;
-  const uint32_t nt0 = (part * 4 + wave) * NTW;  // first 32-word column tile of this wave
-  const uint32_t nkb = T.nkb;
-
-  // items (column tile nt, K-block kb); the next item's operands load while
-  // the current one computes
-  uint32_t xn[16];
-  int4v an0, an1;
-  uint32_t nt = 0, kb = 0;
-  mfma_load<ROT>(in_r, kc, 0, h, nt0 * 32 + r, xn);
-  an0 = P.adata[(size_t)T.a_off * 64 + lane];
-  an1 = P.adata[(size_t)(T.a_off + 1) * 64 + lane];
-  int16v acc0 = {}, acc1 = {}, acc2 = {}, acc3 = {};
-  while (true) {
-    uint32_t x[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) x[j] = xn[j];
-    const int4v a0 = an0, a1 = an1;
-    const uint32_t cnt = nt, ckb = kb;
-    bool more = true;
-    if (kb + 1 < nkb) {
-      ++kb;
-    } else if (nt + 1 < (uint32_t)NTW) {
-      ++nt;
-      kb = 0;
-    } else {
-      more = false;
-    }
-    if (more) {
-      mfma_load<ROT>(in_r, kc, kb, h, (nt0 + nt) * 32 + r, xn);
-      if (nkb > 1) {
-        an0 = P.adata[(size_t)(T.a_off + 2 * kb) * 64 + lane];
-        an1 = P.adata[(size_t)(T.a_off + 2 * kb + 1) * 64 + lane];
-      }
-    }
-    int4v pl[4];
-    byte_planes(x, pl);
-    acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[0], acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[1], acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[2], acc2, 0, 0, 0);
-    acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[3], acc3, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[0], acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[1], acc2, 0, 0, 0);
-    acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[2], acc3, 0, 0, 0);
-    if (ckb + 1 == nkb) {
-      const uint32_t n0 = (nt0 + cnt) * 32;  // uniform
-      const uint32_t word = n0 + r;
-      const uint32_t voff = 4 * h * kCtWords * 4u + word * 4u;
-      // b polynomial: bias on samples < NS, words >= RS hidden (uniform per tile
-      // of 32 words except where NS or RS falls inside it)
-      const bool b_poly = n0 >= kN;
-      const uint32_t jb = word - kN;
-      const bool lane_bias = b_poly && jb < P.ns;
-      const bool lane_keep = !(b_poly && jb >= P.rs);
-      uint32_t o[16];
-#pragma unroll
-      for (int v = 0; v < 16; ++v)
-        o[v] = (uint32_t)acc0[v] + ((uint32_t)acc1[v] << 8) + ((uint32_t)acc2[v] << 16) +
-               ((uint32_t)acc3[v] << 24);
-      if (b_poly && n0 - kN < P.ns) {
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const uint32_t m = 8 * (v / 4) + 4 * h + (v % 4);
-          o[v] += lane_bias ? P.rowc[T.row_off + m] : 0u;
-        }
-      }
-      if (b_poly && n0 + 31 - kN >= P.rs) {
-#pragma unroll
-        for (int v = 0; v < 16; ++v) o[v] = lane_keep ? o[v] : 0u;
-      }
-      if (T.nrows == kMfmaRows) {
-#pragma unroll
-        for (int v = 0; v < 16; ++v)
-          __builtin_amdgcn_raw_buffer_store_b32(
-              (int)o[v], out_r, voff, (int)((T.slot0 + 8 * (v / 4) + (v % 4)) * kCtWords * 4u),
-              IHE_STORE_POLICY);
-      } else {
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const uint32_t m = 8 * (v / 4) + 4 * h + (v % 4);
-          if (m < T.nrows)
-            __builtin_amdgcn_raw_buffer_store_b32(
-                (int)o[v], out_r, voff, (int)((T.slot0 + 8 * (v / 4) + (v % 4)) * kCtWords * 4u),
-                IHE_STORE_POLICY);
-        }
-      }
-      acc0 = acc1 = acc2 = acc3 = int16v{};
-    }
-    if (!more) break;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Single-K-block MFMA kernel: tiles of 32*RB rows over at most 32 columns (a
-// 5-neighbour band fits 64-row tiles almost everywhere). A wave keeps the
-// tile's A operands in registers and walks NTW column tiles of 32 words; each
-// step's B operand (16 loads per lane) feeds RB row blocks of 32 outputs, so
-// the input words read per output word fall to 1/RB. Every step issues the
-// same vector-memory sequence, 16 loads (the next step's columns; the last
-// step reloads its own) followed by 16*RB stores, so the wait for the loads
-// leaves the stores in flight while the next step computes. Rows past the
-// tile's end store to an offset beyond the buffer's range, which the hardware
-// drops. Row biases live in LDS. Blocks are renumbered so that each XCD
-// (blocks are dealt round-robin over the 8 XCDs) walks a contiguous run of
-// tiles, whose windows overlap in that XCD's L2.
-constexpr uint32_t kK1Waves = 4;
-constexpr uint32_t kDropOffset = 0x80000000u;  // >= every segment's byte size
-
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
-  const uint32_t per = n / 8;
-  return b < 8 * per ? (b % 8) * per + b / 8 : b;
-}
-
-template <bool ROT>
-__device__ __forceinline__ void k1_load(__amdgpu_buffer_rsrc_t in_r, const uint32_t (&cw)[16],
-                                        uint32_t word, uint32_t (&x)[16]) {
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    if constexpr (!ROT) {
-      x[j] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(in_r, cw[j] + word * 4u, 0, 0);
-    } else {
-      const uint32_t t = (word & (kN - 1)) + ((cw[j] >> 2) & (kN - 1));
-      const uint32_t v = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
-          in_r, (cw[j] & ~0x1FFFu) + ((t & (kN - 1)) << 2) + (word & kN) * 4u, 0, 0);
-      x[j] = t >= kN ? 0u - v : v;
-    }
-  }
-}
-
-template <bool ROT, int NTW, int RB>
-__global__ __launch_bounds__(64 * kK1Waves) void k_cloud_mfma1(MfmaParams P) {
-  constexpr uint32_t wg_per_tile = kCtWords / (kK1Waves * NTW * 32);
-  constexpr uint32_t rows = kMfmaRows * RB;
-  __shared__ uint32_t s_bias[rows];
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t tile = P.tile_base + blk / wg_per_tile;
-  const uint32_t part = blk % wg_per_tile;
-  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const uint32_t h = lane / 32, r = lane % 32;
-  const MTileDev T = __builtin_bit_cast(MTileDev, *(const IHE_CONSTANT u32x8 *)(P.tiles + tile));
-  const __amdgpu_buffer_rsrc_t in_r =
-      __builtin_amdgcn_make_buffer_rsrc((void *)P.in, (short)0, (int)P.in_bytes, kRsrcWord3);
-  const __amdgpu_buffer_rsrc_t out_r =
-      __builtin_amdgcn_make_buffer_rsrc((void *)P.out, (short)0, (int)P.out_bytes, kRsrcWord3);
-  if (threadIdx.x < rows) s_bias[threadIdx.x] = P.rowc[T.row_off + threadIdx.x];
-  // the lane's 16 column words: columns 16*h + j
-  uint32_t cw[16];
-  {
-    const IHE_CONSTANT uint32_t *kc = cptr(P.kcols) + T.kcol_off;
-    const u32x16 cl = *(const IHE_CONSTANT u32x16 *)kc;
-    const u32x16 chh = *(const IHE_CONSTANT u32x16 *)(kc + 16);
-#pragma unroll
-    for (int j = 0; j < 16; ++j) cw[j] = h ? chh[j] : cl[j];
-  }
-  int4v a[RB][2];
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb) {
-    a[rb][0] = P.adata[(size_t)(T.a_off + 2 * rb) * 64 + lane];
-    a[rb][1] = P.adata[(size_t)(T.a_off + 2 * rb + 1) * 64 + lane];
-  }
-  const uint32_t nt0 = (part * kK1Waves + wave) * NTW;  // first column tile of this wave
-  // store offsets: the lane's row half, plus an out-of-range offset for rows at
-  // or past nrows; per-value row offsets are uniform
-  const uint32_t row_base = T.slot0 * kCtWords * 4u;
-  const bool full = T.nrows == rows;
-  __syncthreads();
-
-  // one step: column tile nt from x, stores issued last
-  auto step = [&](const uint32_t (&x)[16], uint32_t nt) {
-    int4v pl[4];
-    byte_planes(x, pl);
-    const uint32_t n0 = (nt0 + nt) * 32;  // uniform
-    const uint32_t word = n0 + r;
-    const uint32_t voff = word * 4u + 4 * h * kCtWords * 4u;
-    // b polynomial: bias on samples < NS, words >= RS hidden; both uniform per
-    // column tile except where NS or RS falls inside it
-    const bool b_poly = n0 >= kN;
-    const bool bias_tile = b_poly && n0 - kN < P.ns, lane_bias = word - kN < P.ns;
-    const bool hide_tile = b_poly && n0 + 31 - kN >= P.rs, keep = word - kN < P.rs;
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-      int16v acc0 = {}, acc1 = {}, acc2 = {}, acc3 = {};
-      acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][0], pl[0], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][0], pl[1], acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][0], pl[2], acc2, 0, 0, 0);
-      acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][0], pl[3], acc3, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][1], pl[0], acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][1], pl[1], acc2, 0, 0, 0);
-      acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rb][1], pl[2], acc3, 0, 0, 0);
-      uint32_t o[16];
-#pragma unroll
-      for (int v = 0; v < 16; ++v)
-        o[v] = (uint32_t)acc0[v] + ((uint32_t)acc1[v] << 8) + ((uint32_t)acc2[v] << 16) +
-               ((uint32_t)acc3[v] << 24);
-      if (bias_tile) {
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const u32x4 bv = *(const u32x4 *)(s_bias + 32 * rb + 8 * g + 4 * h);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) o[4 * g + q] += lane_bias ? bv[q] : 0u;
-        }
-      }
-      if (hide_tile) {
-#pragma unroll
-        for (int v = 0; v < 16; ++v) o[v] = keep ? o[v] : 0u;
-      }
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const uint32_t m = 32 * rb + 8 * (v / 4) + (v % 4);  // row of lanes 0..31
-        __builtin_amdgcn_raw_buffer_store_b32(
-            (int)o[v], out_r, full || m + 4 * h < T.nrows ? voff : kDropOffset,
-            (int)(row_base + m * kCtWords * 4u), IHE_STORE_POLICY);
-      }
-    }
-  };
-
-  // The first step is peeled so that the loop is entered, like it is re-entered,
-  // with 16 loads followed by the stores outstanding.
-  uint32_t x[16], xn[16];
-  k1_load<ROT>(in_r, cw, nt0 * 32 + r, x);
-  k1_load<ROT>(in_r, cw, (nt0 + (NTW > 1 ? 1 : 0)) * 32 + r, xn);
-  step(x, 0);
-  for (uint32_t nt = 1; nt < (uint32_t)NTW; ++nt) {
-#pragma unroll
-    for (int j = 0; j < 16; ++j) x[j] = xn[j];
-    const uint32_t ntn = nt + 1 < (uint32_t)NTW ? nt + 1 : nt;
-    k1_load<ROT>(in_r, cw, (nt0 + ntn) * 32 + r, xn);
-    step(x, nt);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Warp-specialized, LDS-staged MFMA kernel for tiles of at most 64 columns.
-// Persistent workgroups of five waves walk (tile, 256-word slice) items; one
-// workgroup keeps one slice, so each XCD streams one slice of every tile and
-// consecutive tiles share their window's columns in that XCD's L2. A producer
-// wave copies the next item's columns (one 16-byte-per-lane LDS DMA load per
-// column), A operands and biases into the other half of a double-buffered
-// LDS region and waits for its own loads only; four consumer waves compute
-// the current item from LDS with the MFMA recipe above and issue the output
-// stores. Consumers never wait on vector memory, so the store stream is not
-// throttled by load latency.
-constexpr uint32_t kWsCons = 4;                        // consumer waves
-constexpr uint32_t kWsNtc = 2;                         // 32-word column tiles per consumer
-constexpr uint32_t kWsSlice = 32 * kWsCons * kWsNtc;   // 256 words per item
-constexpr uint32_t kWsHalf = 16 * kWsSlice + 32;       // words per half K-block (32-word bank skew)
-constexpr uint32_t kWsKbWords = 2 * kWsHalf;           // LDS words per K-block of 32 columns
-constexpr uint32_t kWsMaxKb = 2;                       // K-blocks per tile (64 columns)
-constexpr uint32_t kWsGridPerCu = 2;
-
-struct WsParams {
-  MfmaParams m;
-  uint32_t ntiles;      // tiles of this segment
-  uint32_t nkb_max;     // K-blocks reserved per LDS buffer
-  uint32_t buf_words;   // words per LDS buffer
-  uint32_t adata_bytes, rowc_bytes;
-};
-
-// LDS word offsets inside one buffer.
-__device__ __forceinline__ uint32_t ws_col(uint32_t kb, uint32_t j) {
-  return kb * kWsKbWords + (j >> 4) * kWsHalf + (j & 15) * kWsSlice;
-}
-__device__ __forceinline__ uint32_t ws_a(const WsParams &W, uint32_t blk) {
-  return W.nkb_max * kWsKbWords + blk * 256;
-}
-__device__ __forceinline__ uint32_t ws_bias(const WsParams &W) {
-  return W.nkb_max * (kWsKbWords + 512);
-}
-
-template <bool ROT>
-__device__ __forceinline__ void ws_stage(const WsParams &W, __amdgpu_buffer_rsrc_t in_r,
-                                         __amdgpu_buffer_rsrc_t a_r, __amdgpu_buffer_rsrc_t b_r,
-                                         uint32_t *buf, uint32_t tile, uint32_t slice,
-                                         uint32_t lane) {
-  const MTileDev T = __builtin_bit_cast(MTileDev, *(const IHE_CONSTANT u32x8 *)(W.m.tiles + tile));
-  const IHE_CONSTANT uint32_t *kc = cptr(W.m.kcols) + T.kcol_off;
-  for (uint32_t kb = 0; kb < T.nkb; ++kb) {
-    const u32x16 c0 = *(const IHE_CONSTANT u32x16 *)(kc + 32 * kb);
-    const u32x16 c1 = *(const IHE_CONSTANT u32x16 *)(kc + 32 * kb + 16);
-#pragma unroll
-    for (int j = 0; j < 32; ++j) {
-      const uint32_t cw = j < 16 ? c0[j] : c1[j - 16];
-      uint32_t *dst = buf + ws_col(kb, (uint32_t)j);
-      if constexpr (!ROT) {
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            in_r, (__attribute__((address_space(3))) void *)dst, 16,
-            (slice * kWsSlice + 4 * lane) * 4u, (int)cw, 0, 0);
-      } else {
-        // rotated words land as read; consumers negate the wrapped ones
-        const uint32_t sh = (cw >> 2) & (kN - 1);
-#pragma unroll
-        for (uint32_t q = 0; q < kWsSlice / 64; ++q) {
-          const uint32_t word = slice * kWsSlice + q * 64 + lane;
-          const uint32_t t = (word & (kN - 1)) + sh;
-          __builtin_amdgcn_raw_ptr_buffer_load_lds(
-              in_r, (__attribute__((address_space(3))) void *)(dst + q * 64), 4,
-              ((word & kN) + (t & (kN - 1))) * 4u, (int)(cw & ~0x1FFFu), 0, 0);
-        }
-      }
-    }
-  }
-  for (uint32_t blk = 0; blk < 2 * T.nkb; ++blk)
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(
-        a_r, (__attribute__((address_space(3))) void *)(buf + ws_a(W, blk)), 16, lane * 16u,
-        (int)((T.a_off + blk) * 1024u), 0, 0);
-  __builtin_amdgcn_raw_ptr_buffer_load_lds(b_r, (__attribute__((address_space(3))) void *)(buf + ws_bias(W)),
-                                           4, lane * 4u, (int)(T.row_off * 4u), 0, 0);
-}
-
-template <bool ROT>
-__device__ __forceinline__ void ws_compute(const WsParams &W, __amdgpu_buffer_rsrc_t out_r,
-                                           const uint32_t *buf, uint32_t tile, uint32_t slice,
-                                           uint32_t wave, uint32_t lane) {
-  const MTileDev T = __builtin_bit_cast(MTileDev, *(const IHE_CONSTANT u32x8 *)(W.m.tiles + tile));
-  const uint32_t h = lane / 32, r = lane % 32;
-  const IHE_CONSTANT uint32_t *kc = cptr(W.m.kcols) + T.kcol_off;
-#pragma unroll
-  for (uint32_t i = 0; i < kWsNtc; ++i) {
-    const uint32_t nl = wave * kWsNtc + i;            // column tile within the slice
-    const uint32_t n0 = slice * kWsSlice + nl * 32;   // first word of the column tile
-    const uint32_t word = n0 + r;
-    int16v acc0 = {}, acc1 = {}, acc2 = {}, acc3 = {};
-    for (uint32_t kb = 0; kb < T.nkb; ++kb) {
-      const int4v a0 = *(const int4v *)(buf + ws_a(W, 2 * kb) + 4 * lane);
-      const int4v a1 = *(const int4v *)(buf + ws_a(W, 2 * kb + 1) + 4 * lane);
-      const uint32_t *col = buf + ws_col(kb, 16 * h) + nl * 32 + r;
-      uint32_t x[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) x[j] = col[j * kWsSlice];
-      if constexpr (ROT) {
-        const u32x16 c0 = *(const IHE_CONSTANT u32x16 *)(kc + 32 * kb);
-        const u32x16 c1 = *(const IHE_CONSTANT u32x16 *)(kc + 32 * kb + 16);
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const uint32_t cw = h ? c1[j] : c0[j];
-          const uint32_t t = (word & (kN - 1)) + ((cw >> 2) & (kN - 1));
-          x[j] = t >= kN ? 0u - x[j] : x[j];
-        }
-      }
-      int4v pl[4];
-      byte_planes(x, pl);
-      acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[0], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[1], acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[2], acc2, 0, 0, 0);
-      acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[3], acc3, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[0], acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[1], acc2, 0, 0, 0);
-      acc3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[2], acc3, 0, 0, 0);
-    }
-    const uint32_t voff = 4 * h * kCtWords * 4u + word * 4u;
-    const bool b_poly = n0 >= kN;
-    const uint32_t jb = word - kN;
-    uint32_t o[16];
-#pragma unroll
-    for (int v = 0; v < 16; ++v)
-      o[v] = (uint32_t)acc0[v] + ((uint32_t)acc1[v] << 8) + ((uint32_t)acc2[v] << 16) +
-             ((uint32_t)acc3[v] << 24);
-    if (b_poly && n0 - kN < W.m.ns) {
-      const bool lane_bias = jb < W.m.ns;
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const u32x4 bv = *(const u32x4 *)(buf + ws_bias(W) + 8 * g + 4 * h);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) o[4 * g + q] += lane_bias ? bv[q] : 0u;
-      }
-    }
-    if (b_poly && n0 + 31 - kN >= W.m.rs) {
-      const bool keep = jb < W.m.rs;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) o[v] = keep ? o[v] : 0u;
-    }
-    if (T.nrows == kMfmaRows) {
-#pragma unroll
-      for (int v = 0; v < 16; ++v)
-        __builtin_amdgcn_raw_buffer_store_b32(
-            (int)o[v], out_r, voff, (int)((T.slot0 + 8 * (v / 4) + (v % 4)) * kCtWords * 4u),
-            IHE_STORE_POLICY);
-    } else {
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const uint32_t m = 8 * (v / 4) + 4 * h + (v % 4);
-        if (m < T.nrows)
-          __builtin_amdgcn_raw_buffer_store_b32(
-              (int)o[v], out_r, voff, (int)((T.slot0 + 8 * (v / 4) + (v % 4)) * kCtWords * 4u),
-              IHE_STORE_POLICY);
-      }
-    }
-  }
-}
-
-template <bool ROT>
-__global__ __launch_bounds__(64 * (kWsCons + 1)) void k_cloud_mfma_ws(WsParams W) {
-  extern __shared__ __attribute__((aligned(16))) uint32_t ws_lds[];
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const bool producer = wave == kWsCons;
-  const uint32_t slice = blockIdx.x & 7u;  // grid is a multiple of 8
-  const uint32_t step = gridDim.x >> 3;
-  const uint32_t t_end = W.m.tile_base + W.ntiles;
-  uint32_t tile = W.m.tile_base + (blockIdx.x >> 3);
-  const __amdgpu_buffer_rsrc_t in_r =
-      __builtin_amdgcn_make_buffer_rsrc((void *)W.m.in, (short)0, (int)W.m.in_bytes, kRsrcWord3);
-  const __amdgpu_buffer_rsrc_t out_r =
-      __builtin_amdgcn_make_buffer_rsrc((void *)W.m.out, (short)0, (int)W.m.out_bytes, kRsrcWord3);
-  const __amdgpu_buffer_rsrc_t a_r =
-      __builtin_amdgcn_make_buffer_rsrc((void *)W.m.adata, (short)0, (int)W.adata_bytes, kRsrcWord3);
-  const __amdgpu_buffer_rsrc_t b_r =
-      __builtin_amdgcn_make_buffer_rsrc((void *)W.m.rowc, (short)0, (int)W.rowc_bytes, kRsrcWord3);
-  if (tile >= t_end) return;  // uniform for the whole workgroup
-  if (producer) {
-    ws_stage<ROT>(W, in_r, a_r, b_r, ws_lds, tile, slice, lane);
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the staged item has landed in LDS
-  }
-  __syncthreads();
-  for (uint32_t k = 0; tile < t_end; tile += step, ++k) {
-    uint32_t *cur = ws_lds + (k & 1) * W.buf_words;
-    if (producer) {
-      if (tile + step < t_end) {
-        ws_stage<ROT>(W, in_r, a_r, b_r, ws_lds + ((k + 1) & 1) * W.buf_words, tile + step, slice,
-                      lane);
-        __builtin_amdgcn_s_waitcnt(0x0F70);
-      }
-    } else {
-      ws_compute<ROT>(W, out_r, cur, tile, slice, wave, lane);
-    }
-    __syncthreads();
-  }
-}
-
 }  // namespace ihe
 
 using namespace ihe;
 
-namespace {
-
-struct Segment {
-  uint32_t tile_begin, tile_end;  // tiles [begin, end)
-  uint32_t in_base, out_base;     // first input ciphertext / output slot addressed
-};
-
-}  // namespace
-
-enum { kModeDirect = 0, kModeStaged = 1, kModeMfma = 2, kModeMfmaWs = 3, kModeMfma1 = 4 };
-inline bool is_mfma_mode(int m) { return m == kModeMfma || m == kModeMfmaWs || m == kModeMfma1; }
-
-struct ihe_band {
-  ihe_ctx *ctx = nullptr;
-  ihe_layout layout{};
-  uint32_t num_in = 0, num_out = 0;
-  int variant = IHE_VARIANT_VALU;
-  int mode = 0;  // kModeDirect, kModeStaged or kModeMfma
-  int wpl = 1;
-  uint32_t row_blocks = 1;  // 32-row MFMA blocks per tile
-  uint32_t num_groups = 0, num_tiles = 0, num_cols = 0, nnz = 0, max_window = 0;
-  uint32_t lds_bytes = 0, meta_lds = 0, max_kb = 0;
-  size_t adata_bytes = 0, rowc_bytes = 0;
-  std::vector<Segment> segments;
-  void *dmem = nullptr;
-  size_t dbytes = 0;
-  // direct layout
-  GroupDev *d_grp = nullptr;
-  uint32_t *d_tile_off = nullptr, *d_cols = nullptr, *d_oslot = nullptr, *d_obias = nullptr;
-  int32_t *d_wts = nullptr;
-  // staged layout
-  TileDev *d_tiles = nullptr;
-  uint32_t *d_stage = nullptr, *d_meta = nullptr;
-  // MFMA layout
-  MTileDev *d_mtiles = nullptr;
-  uint32_t *d_kcols = nullptr, *d_rowc = nullptr;
-  int4v *d_adata = nullptr;
-  // variances
-  uint32_t *d_var_ptr = nullptr, *d_var_ct = nullptr;
-  int32_t *d_var_w2 = nullptr;
-};
 
 namespace {
 
@@ -1044,12 +467,14 @@ struct HostBand {
   std::vector<MTileDev> mtiles;
   std::vector<uint32_t> kcols, rowc;
   std::vector<int4v> adata;
-  uint32_t max_kb = 0, row_blocks = 1;
+  MfmaShape shape;
+  std::vector<MfmaShape> candidates;  // feasible MFMA shapes, for the autotuner
   // all layouts
   std::vector<Segment> segments;
-  std::vector<uint32_t> var_ptr, var_ct, bias;
+  std::vector<uint32_t> var_ptr, var_col, bias;
   std::vector<int32_t> var_w2;
-  uint32_t max_window = 0;
+  uint32_t max_window = 0, max_bidx = 0;
+  bool has_cols = false;
   // grouping
   std::vector<HostGroup> hg;
   std::vector<uint32_t> gcol, gout;
@@ -1085,12 +510,14 @@ int build_groups(const ihe_layout &L, uint32_t num_in, const ihe_model_csr &M, b
       if (!wide && (w < -32768 || w > 32767))
         return fail(IHE_ERR_UNSUPPORTED, "weight outside int16 for the dot2 variant");
       row.emplace_back(c, w);
-      if (c % R == 0) {
-        H.var_ct.push_back(c / R);
-        H.var_w2.push_back((int32_t)((uint32_t)w * (uint32_t)w));
-      }
+      // tLweAddMulTo's variance update, (int32)(w*w) * var; only region-0 columns
+      // reach the output (the rotated regions carry none, eval/idash.cpp:786, :835)
+      H.var_col.push_back(c);
+      H.var_w2.push_back((int32_t)((uint32_t)w * (uint32_t)w));
+      H.max_bidx = std::max(H.max_bidx, c);
+      H.has_cols = true;
     }
-    H.var_ptr[o + 1] = (uint32_t)H.var_ct.size();
+    H.var_ptr[o + 1] = (uint32_t)H.var_col.size();
     std::sort(row.begin(), row.end());
     for (size_t k = 1; k < row.size(); ++k)
       if (row[k].first == row[k - 1].first)
@@ -1355,10 +782,14 @@ void layout_staged(const ihe_layout &L, uint32_t tile_outputs, HostBand &H) {
 
 // MFMA layout: tiles of 32 consecutive output slots over the union of their
 // columns. Returns false when a tile's union exceeds kMfmaMaxKb blocks.
-bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, uint32_t row_blocks,
-                 HostBand &H) {
-  const uint32_t rows = kMfmaRows * row_blocks;
-  const uint32_t R = L.num_regions, RS = L.region_size;
+// MFMA layout: tiles of up to 32*RB consecutive rows whose column union fits
+// nkb K-blocks of 32 columns. Column lists hold big indices, so the tiling is
+// independent of the packing layout; segments bound the big-index and output
+// ranges so that every launch addresses its buffers with 32-bit offsets.
+// Returns false when a single row needs more than nkb K-blocks.
+bool layout_mfma(uint32_t num_out, const MfmaShape &shape, HostBand &H) {
+  const uint32_t rows = kMfmaRows * (uint32_t)shape.rb;
+  const uint32_t kb_budget = (uint32_t)shape.nkb;
   // output slot -> (group, row within the group)
   std::vector<uint32_t> og(num_out), orow(num_out);
   for (uint32_t g = 0; g < H.hg.size(); ++g)
@@ -1366,7 +797,6 @@ bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, uint
       og[H.gout[H.hg[g].out_begin + i]] = g;
       orow[H.gout[H.hg[g].out_begin + i]] = i;
     }
-  // tiles: up to `rows` consecutive rows whose column union fits kb_budget blocks
   std::vector<std::vector<uint32_t>> U;
   std::vector<uint32_t> t_o0, t_rows;
   std::vector<uint32_t> merged;
@@ -1395,17 +825,16 @@ bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, uint
     t_rows.push_back(o - o0);
   }
   const uint32_t ntiles = (uint32_t)U.size();
-  std::vector<std::array<uint32_t, 4>> rng(ntiles);
+  SegmentBuilder sb{H.segments};
   for (uint32_t t = 0; t < ntiles; ++t) {
     const std::vector<uint32_t> &u = U[t];
-    rng[t] = {u.empty() ? UINT32_MAX : u.front() / R, u.empty() ? 0 : u.back() / R, t_o0[t],
-              t_o0[t] + t_rows[t] - 1};
-    if (!u.empty() && rng[t][1] - rng[t][0] >= kSegCts) return false;
+    if (!u.empty() && u.back() - u.front() >= kSegCts) return false;
+    sb.add(t, u.empty() ? UINT32_MAX : u.front(), u.empty() ? 0 : u.back(), t_o0[t],
+           t_o0[t] + t_rows[t] - 1);
   }
-  SegmentBuilder sb{H.segments};
-  for (uint32_t t = 0; t < ntiles; ++t) sb.add(t, rng[t][0], rng[t][1], rng[t][2], rng[t][3]);
   sb.close(ntiles);
 
+  H.shape = shape;
   for (const Segment &seg : H.segments) {
     for (uint32_t t = seg.tile_begin; t < seg.tile_end; ++t) {
       const std::vector<uint32_t> &u = U[t];
@@ -1418,9 +847,9 @@ bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, uint
       D.row_off = (uint32_t)H.rowc.size();
       D.slot0 = o0 - seg.out_base;
       D.nrows = nrows;
-      const uint32_t fill = u.empty() ? seg.in_base * R : u.back();
-      for (uint32_t k = 0; k < 32 * nkb; ++k)
-        H.kcols.push_back(col_word(k < u.size() ? u[k] : fill, R, RS, seg.in_base));
+      // padding columns repeat a real one (or the segment base) with weight 0
+      const uint32_t fill = u.empty() ? seg.in_base : u.back();
+      for (uint32_t k = 0; k < 32 * nkb; ++k) H.kcols.push_back(k < u.size() ? u[k] : fill);
       // dense weights of the tile: W[m][k]
       std::vector<int32_t> W((size_t)rows * 32 * nkb, 0);
       for (uint32_t m = 0; m < nrows; ++m) {
@@ -1433,24 +862,23 @@ bool layout_mfma(const ihe_layout &L, uint32_t num_out, uint32_t kb_budget, uint
         }
       }
       // A operand blocks in (row block, K-block, weight byte) order
-      for (uint32_t rb = 0; rb < row_blocks; ++rb)
-      for (uint32_t kb = 0; kb < nkb; ++kb)
-        for (uint32_t hp = 0; hp < 2; ++hp)
-          for (uint32_t lane = 0; lane < 64; ++lane) {
-            int4v a;
-            int8_t bytes[16];
-            for (uint32_t j = 0; j < 16; ++j) {
-              const uint32_t m = 32 * rb + lane % 32;
-              const int32_t w = W[(size_t)m * 32 * nkb + 32 * kb + 16 * (lane / 32) + j];
-              const int32_t w0 = (int8_t)(w & 0xFF);
-              bytes[j] = (int8_t)(hp == 0 ? w0 : (w - w0) / 256);
+      for (uint32_t rb = 0; rb < (uint32_t)shape.rb; ++rb)
+        for (uint32_t kb = 0; kb < nkb; ++kb)
+          for (uint32_t hp = 0; hp < 2; ++hp)
+            for (uint32_t lane = 0; lane < 64; ++lane) {
+              int8_t bytes[16];
+              for (uint32_t j = 0; j < 16; ++j) {
+                const uint32_t m = 32 * rb + lane % 32;
+                const int32_t w = W[(size_t)m * 32 * nkb + 32 * kb + 16 * (lane / 32) + j];
+                const int32_t w0 = (int8_t)(w & 0xFF);
+                bytes[j] = (int8_t)(hp == 0 ? w0 : (w - w0) / 256);
+              }
+              int4v a;
+              std::memcpy(&a, bytes, 16);
+              H.adata.push_back(a);
             }
-            std::memcpy(&a, bytes, 16);
-            H.adata.push_back(a);
-          }
       for (uint32_t m = 0; m < rows; ++m) H.rowc.push_back(m < nrows ? H.bias[o0 + m] : 0u);
       H.mtiles.push_back(D);
-      H.max_kb = std::max(H.max_kb, nkb);
       H.cols.resize(H.cols.size() + u.size());  // counts tile columns for the band info
     }
   }
@@ -1473,9 +901,103 @@ uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi) {
   return dflt;
 }
 
+// Number of tiles `shape` cuts the outputs into (0 when a row does not fit).
+uint32_t count_tiles(uint32_t num_out, const MfmaShape &shape, const HostBand &H) {
+  const uint32_t rows = kMfmaRows * (uint32_t)shape.rb, budget = 32 * (uint32_t)shape.nkb;
+  std::vector<uint32_t> og(num_out);
+  for (uint32_t g = 0; g < H.hg.size(); ++g)
+    for (uint32_t i = 0; i < H.hg[g].nout; ++i) og[H.gout[H.hg[g].out_begin + i]] = g;
+  std::vector<uint32_t> u, merged;
+  uint32_t tiles = 0;
+  for (uint32_t o = 0; o < num_out;) {
+    const uint32_t o0 = o;
+    u.clear();
+    uint32_t last_g = UINT32_MAX;
+    while (o < num_out && o - o0 < rows) {
+      const uint32_t g = og[o];
+      if (g != last_g) {
+        const HostGroup &hg = H.hg[g];
+        merged.clear();
+        std::set_union(u.begin(), u.end(), H.gcol.begin() + hg.col_begin,
+                       H.gcol.begin() + hg.col_begin + hg.ncol, std::back_inserter(merged));
+        if (merged.size() > budget) {
+          if (o == o0) return 0;
+          break;
+        }
+        u.swap(merged);
+        last_g = g;
+      }
+      ++o;
+    }
+    ++tiles;
+  }
+  return tiles;
+}
+
+// MFMA tile shape: IHE_CLOUD_SHAPE="rb,nkb" if given, else the candidate with
+// the least estimated step work, tiles * (nkb + 1.5 * rb): a column-tile step
+// issues 16 loads per K-block and 16 stores per row block. The kernel
+// autotuner (ihe_band_tune) refines this choice by measurement.
+bool pick_mfma_shape(uint32_t num_out, HostBand &H, MfmaShape &shape) {
+  shape.ntw = (int)env_u32("IHE_CLOUD_NTW", 4, 4, 16);
+  if (shape.ntw != 8 && shape.ntw != 16) shape.ntw = 4;
+  if (const char *e = std::getenv("IHE_CLOUD_SHAPE")) {
+    int rb = 0, nkb = 0;
+    if (std::sscanf(e, "%d,%d", &rb, &nkb) == 2 && (rb == 1 || (rb == 2 && nkb == 1)) && nkb >= 1 &&
+        nkb <= (int)kMfmaMaxNkb) {
+      shape.rb = rb, shape.nkb = nkb;
+      return count_tiles(num_out, shape, H) > 0;
+    }
+  }
+  const int kmin = std::max<int>(1, (int)((H.max_window + 31) / 32));
+  std::vector<MfmaShape> cands;
+  if (kmin == 1) cands.push_back({2, 1, shape.ntw}), cands.push_back({1, 1, shape.ntw});
+  for (int k = std::max(2, kmin); k <= std::min<int>(kmin + 2, (int)kMfmaMaxNkb); ++k)
+    cands.push_back({1, k, shape.ntw});
+  double best = 0;
+  bool found = false;
+  for (const MfmaShape &c : cands) {
+    const uint32_t t = count_tiles(num_out, c, H);
+    if (t == 0) continue;
+    H.candidates.push_back(c);
+    const double cost = (double)t * (c.nkb + 1.5 * c.rb);
+    if (!found || cost < best) best = cost, shape = c, found = true;
+  }
+  return found || num_out == 0;
+}
+
+// Rewrites the per-row variance entries (var_ptr / var_col / var_w2, row
+// order) into the kernel's layout: per block of 64 outputs, entry j of every
+// row side by side, rows padded with kVarPad to the block's longest row;
+// var_ptr becomes the per-block offsets in units of 64 entries.
+void interleave_variances(HostBand &H) {
+  const uint32_t num_out = (uint32_t)H.var_ptr.size() - 1;
+  const uint32_t nblk = (num_out + 63) / 64;
+  std::vector<uint32_t> off(nblk + 1, 0), col;
+  std::vector<int32_t> w2;
+  for (uint32_t b = 0; b < nblk; ++b) {
+    uint32_t len = 0;
+    for (uint32_t o = 64 * b; o < std::min(num_out, 64 * b + 64); ++o)
+      len = std::max(len, H.var_ptr[o + 1] - H.var_ptr[o]);
+    off[b + 1] = off[b] + len;
+  }
+  col.assign((size_t)off[nblk] * 64, kVarPad);
+  w2.assign((size_t)off[nblk] * 64, 0);
+  for (uint32_t o = 0; o < num_out; ++o) {
+    const uint32_t b = o / 64, l = o % 64;
+    for (uint32_t e = H.var_ptr[o], j = 0; e < H.var_ptr[o + 1]; ++e, ++j) {
+      col[((size_t)off[b] + j) * 64 + l] = H.var_col[e];
+      w2[((size_t)off[b] + j) * 64 + l] = H.var_w2[e];
+    }
+  }
+  H.var_ptr.swap(off);
+  H.var_col.swap(col);
+  H.var_w2.swap(w2);
+}
+
 // Validates arguments, picks the kernel family and compiles the band on the host.
 int plan_band(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr *model,
-              int variant, HostBand &H) {
+              int variant, HostBand &H, const MfmaShape *force_shape = nullptr) {
   if (!model) return fail(IHE_ERR_ARG, "null model");
   if (int rc = check_layout(layout)) return rc;
   if (model->num_outputs > 0 && (!model->row_ptr || (model->row_ptr[model->num_outputs] > 0 &&
@@ -1506,7 +1028,7 @@ int plan_band(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr
     for (uint32_t e = 0; e < nnz && mfma_ok; ++e)
       if (model->in_bidx[e] != IHE_CONSTANT_BIDX && model->weight[e] > 32639) mfma_ok = false;
   }
-  // kernel choice; IHE_CLOUD_KERNEL=direct|staged|mfma|mfma1|ws|valu overrides it for
+  // kernel family; IHE_CLOUD_KERNEL=direct|staged|valu|mfma overrides it for
   // experiments within what the variant allows
   const char *force_env = std::getenv("IHE_CLOUD_KERNEL");
   const std::string force = force_env ? force_env : "";
@@ -1515,36 +1037,23 @@ int plan_band(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr
   if (variant == IHE_VARIANT_MFMA || (variant == IHE_VARIANT_AUTO && mfma_ok &&
                                       force != "direct" && force != "staged" && force != "valu")) {
     if (!mfma_ok) return fail(IHE_ERR_UNSUPPORTED, "MFMA variant needs weights in [-32768, 32639]");
-    mode = force == "mfma" ? kModeMfma : force == "ws" ? kModeMfmaWs : kModeMfma1;
+    mode = kModeMfma;
   } else if (wide || force == "direct") {
     mode = kModeDirect;
   } else {
     mode = fits_stage ? kModeStaged : kModeDirect;
   }
-  auto clear_mfma = [&H] {
-    H.segments.clear();
-    H.mtiles.clear(), H.kcols.clear(), H.rowc.clear(), H.adata.clear(), H.cols.clear();
-    H.max_kb = 0;
-  };
-  // single-K-block tiles, else tiles within two K-blocks for the staged kernel
-  if (mode == kModeMfma1) H.row_blocks = env_u32("IHE_CLOUD_ROW_BLOCKS", 2, 1, 2);
-  if (mode == kModeMfma1 && !layout_mfma(*layout, model->num_outputs, 1, H.row_blocks, H)) {
-    clear_mfma();
-    mode = kModeMfma;
-  }
-  if (mode == kModeMfmaWs) {
-    const uint32_t budget = H.max_window <= 32 ? 1 : H.max_window <= 64 ? 2 : 0;
-    if (budget == 0 || !layout_mfma(*layout, model->num_outputs, budget, 1, H)) {
-      clear_mfma();
-      mode = kModeMfma;
+  if (mode == kModeMfma) {
+    MfmaShape shape;
+    const bool ok = force_shape ? (shape = *force_shape, count_tiles(model->num_outputs, shape, H) > 0)
+                          : pick_mfma_shape(model->num_outputs, H, shape);
+    if (!ok || !layout_mfma(model->num_outputs, shape, H)) {
+      if (variant == IHE_VARIANT_MFMA)
+        return fail(IHE_ERR_UNSUPPORTED, "an MFMA tile needs more than 256 distinct columns");
+      H.segments.clear();
+      H.mtiles.clear(), H.kcols.clear(), H.rowc.clear(), H.adata.clear(), H.cols.clear();
+      mode = fits_stage ? kModeStaged : kModeDirect;
     }
-  }
-  if (mode == kModeMfma) H.row_blocks = 1;
-  if (mode == kModeMfma && !layout_mfma(*layout, model->num_outputs, kMfmaMaxKb, 1, H)) {
-    if (variant == IHE_VARIANT_MFMA)
-      return fail(IHE_ERR_UNSUPPORTED, "an MFMA tile needs more than 512 distinct columns");
-    clear_mfma();
-    mode = fits_stage ? kModeStaged : kModeDirect;
   }
   H.mode = mode;
   if (mode == kModeStaged) layout_staged(*layout, env_u32("IHE_CLOUD_TILE_OUTPUTS", 48, 1, 4096), H);
@@ -1556,12 +1065,12 @@ void fill_info(const HostBand &H, const ihe_model_csr *model, ihe_band_info *inf
   info->num_outputs = model->num_outputs;
   info->num_groups = (uint32_t)H.hg.size();
   info->num_tiles = H.mode == kModeStaged                          ? (uint32_t)H.tiles.size()
-                   : is_mfma_mode(H.mode) ? (uint32_t)H.mtiles.size()
+                   : H.mode == kModeMfma ? (uint32_t)H.mtiles.size()
                                                                      : (uint32_t)H.tile_off.size() - 1;
   info->num_cols = (uint32_t)H.cols.size();
   info->nnz = model->num_outputs ? model->row_ptr[model->num_outputs] : 0;
   info->variant = H.wide ? IHE_VARIANT_WIDE
-                : is_mfma_mode(H.mode) ? IHE_VARIANT_MFMA
+                : H.mode == kModeMfma ? IHE_VARIANT_MFMA
                                                                  : IHE_VARIANT_VALU;
   info->max_window = H.max_window;
   info->device_bytes = 0;
@@ -1587,14 +1096,16 @@ int ihe_band_plan(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model
   return IHE_OK;
 }
 
-int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
-                    const ihe_model_csr *model, int variant, ihe_band **out) {
-  if (int rc = check_ctx(ctx)) return rc;
-  if (!out) return fail(IHE_ERR_ARG, "null argument");
-  *out = nullptr;
-  HostBand H;
-  if (int rc = plan_band(layout, num_in_cts, model, variant, H)) return rc;
+}  // extern "C"
 
+namespace {
+
+void launch_mfma_band(ihe_ctx *ctx, const ihe_band *band, const ihe_layout &L, uint32_t num_in,
+                      const int32_t *d_in, int32_t *d_out);
+
+// Uploads a compiled band.
+int upload_band(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
+                const ihe_model_csr *model, HostBand &H, ihe_band **out) {
   ihe_band *b = new ihe_band();
   b->wpl = (int)env_u32("IHE_CLOUD_WPL", 1, 1, 4);
   if (b->wpl == 3) b->wpl = 2;
@@ -1613,15 +1124,17 @@ int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
   b->max_window = info.max_window;
   b->segments = H.segments;
   b->meta_lds = H.max_stage * kStageSlice;
-  b->max_kb = H.max_kb;
-  b->row_blocks = H.row_blocks;
+  b->shape = H.shape;
+  b->max_bidx = H.max_bidx;
+  b->has_cols = H.has_cols;
   b->lds_bytes = (b->meta_lds + H.max_meta) * 4u;
 
+  interleave_variances(H);
   size_t bytes = 0;
   for (size_t n : {H.grp.size() * 8, H.tile_off.size(), H.cols.size(), H.oslot.size(),
                    H.obias.size(), H.wts.size(), H.tiles.size() * 8, H.stage.size(), H.meta.size(),
                    H.mtiles.size() * 8, H.kcols.size(), H.rowc.size(), H.adata.size() * 4,
-                   H.var_ptr.size(), H.var_ct.size(), H.var_w2.size()})
+                   H.var_ptr.size(), H.var_col.size(), H.var_w2.size()})
     bytes += align_up(4 * std::max<size_t>(n, 1));
   if (int rc = ihe_malloc(ctx, bytes, &b->dmem)) {
     delete b;
@@ -1650,10 +1163,8 @@ int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
   put(&b->d_kcols, H.kcols);
   put(&b->d_rowc, H.rowc);
   put(&b->d_adata, H.adata);
-  b->adata_bytes = H.adata.size() * sizeof(int4v);
-  b->rowc_bytes = H.rowc.size() * 4;
   put(&b->d_var_ptr, H.var_ptr);
-  put(&b->d_var_ct, H.var_ct);
+  put(&b->d_var_col, H.var_col);
   put(&b->d_var_w2, H.var_w2);
   if (int rc = ihe_memcpy_h2d(ctx, b->dmem, staging.data(), bytes)) {
     (void)ihe_free(ctx, b->dmem);
@@ -1661,6 +1172,87 @@ int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
     return rc;
   }
   *out = b;
+  return IHE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
+                    const ihe_model_csr *model, int variant, ihe_band **out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!out) return fail(IHE_ERR_ARG, "null argument");
+  *out = nullptr;
+  HostBand H;
+  if (int rc = plan_band(layout, num_in_cts, model, variant, H)) return rc;
+  return upload_band(ctx, layout, num_in_cts, model, H, out);
+}
+
+int ihe_band_create_tuned(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
+                          const ihe_model_csr *model, const int32_t *d_in, int32_t *d_out,
+                          uint32_t reps, ihe_band **out, ihe_tune_entry *results,
+                          uint32_t *num_results) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!out) return fail(IHE_ERR_ARG, "null argument");
+  *out = nullptr;
+  const uint32_t cap = num_results ? *num_results : 0;
+  if (num_results) *num_results = 0;
+  HostBand P;
+  if (int rc = plan_band(layout, num_in_cts, model, IHE_VARIANT_MFMA, P)) return rc;
+  if (model->num_outputs == 0 || P.candidates.empty())
+    return upload_band(ctx, layout, num_in_cts, model, P, out);
+  if (!d_out || (num_in_cts > 0 && !d_in)) return fail(IHE_ERR_ARG, "null ciphertext buffer");
+  if (int rc = make_current(ctx)) return rc;
+  reps = std::max<uint32_t>(reps, 1);
+  hipEvent_t e0, e1;
+  IHE_HIP(hipEventCreate(&e0));
+  IHE_HIP(hipEventCreate(&e1));
+  ihe_band *best = nullptr;
+  float best_ms = 0;
+  uint32_t n = 0;
+  int rc = IHE_OK;
+  for (const MfmaShape &cand : std::vector<MfmaShape>(P.candidates)) {
+    HostBand H;
+    if ((rc = plan_band(layout, num_in_cts, model, IHE_VARIANT_MFMA, H, &cand))) break;
+    ihe_band *b = nullptr;
+    if ((rc = upload_band(ctx, layout, num_in_cts, model, H, &b))) break;
+    int best_ntw = 4;
+    float band_ms = 0;
+    for (int ntw : {4, 8, 16}) {
+      b->shape.ntw = ntw;
+      launch_mfma_band(ctx, b, *layout, num_in_cts, d_in, d_out);  // warm-up
+      (void)hipEventRecord(e0, ctx->stream);
+      for (uint32_t i = 0; i < reps; ++i) launch_mfma_band(ctx, b, *layout, num_in_cts, d_in, d_out);
+      (void)hipEventRecord(e1, ctx->stream);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        rc = fail(IHE_ERR_HIP, "autotuner launch failed");
+        break;
+      }
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= (float)reps;
+      if (n < cap) results[n] = {32u * (uint32_t)cand.rb, (uint32_t)cand.nkb, (uint32_t)ntw, ms};
+      ++n;
+      if (ntw == 4 || ms < band_ms) band_ms = ms, best_ntw = ntw;
+    }
+    b->shape.ntw = best_ntw;
+    if (rc == IHE_OK && (!best || band_ms < best_ms)) {
+      if (best) ihe_band_destroy(best);
+      best = b, best_ms = band_ms;
+    } else {
+      ihe_band_destroy(b);
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (num_results) *num_results = std::min(n, cap);
+  if (rc) {
+    if (best) ihe_band_destroy(best);
+    return rc;
+  }
+  *out = best;
   return IHE_OK;
 }
 
@@ -1750,121 +1342,73 @@ void launch_staged(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in, int3
   }
 }
 
-void launch_mfma(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in, int32_t *d_out) {
-  MfmaParams P;
-  P.tiles = band->d_mtiles;
-  P.kcols = band->d_kcols;
-  P.adata = band->d_adata;
-  P.rowc = band->d_rowc;
-  P.ns = band->layout.num_samples;
-  P.rs = band->layout.region_size;
-  const bool rot = band->layout.num_regions > 1;
-  const uint32_t ntw = band->wpl >= 4 ? 16 : band->wpl == 2 ? 8 : 4;  // 32-word tiles per wave
-  const uint32_t wg_per_tile = kCtWords / (4 * ntw * 32);
+// One launch per segment; the tiling is shared by every layout.
+void launch_mfma_band(ihe_ctx *ctx, const ihe_band *band, const ihe_layout &L, uint32_t num_in,
+                      const int32_t *d_in, int32_t *d_out) {
   for (const Segment &s : band->segments) {
-    P.tile_base = s.tile_begin;
-    const uint32_t in_cts = band->num_in > s.in_base ? std::min(band->num_in - s.in_base, kSegCts) : 0u;
-    P.in = in_cts ? d_in + (size_t)s.in_base * kCtWords : nullptr;
-    P.in_bytes = in_cts * kCtWords * 4u;
-    P.out = d_out + (size_t)s.out_base * kCtWords;
-    P.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
-    const dim3 grid((s.tile_end - s.tile_begin) * wg_per_tile), block(256);
-#define IHE_LAUNCH(RT, W) hipLaunchKernelGGL((k_cloud_mfma<RT, W>), grid, block, 0, ctx->stream, P)
-    if (ntw == 16) { if (rot) IHE_LAUNCH(true, 16); else IHE_LAUNCH(false, 16); }
-    else if (ntw == 8) { if (rot) IHE_LAUNCH(true, 8); else IHE_LAUNCH(false, 8); }
-    else { if (rot) IHE_LAUNCH(true, 4); else IHE_LAUNCH(false, 4); }
-#undef IHE_LAUNCH
-  }
-}
-
-void launch_mfma1(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in, int32_t *d_out) {
-  MfmaParams P;
-  P.tiles = band->d_mtiles;
-  P.kcols = band->d_kcols;
-  P.adata = band->d_adata;
-  P.rowc = band->d_rowc;
-  P.ns = band->layout.num_samples;
-  P.rs = band->layout.region_size;
-  const bool rot = band->layout.num_regions > 1;
-  const uint32_t ntw = band->wpl >= 4 ? 16 : band->wpl == 2 ? 8 : 4;  // column tiles per wave
-  const uint32_t wg_per_tile = kCtWords / (kK1Waves * ntw * 32);
-  for (const Segment &s : band->segments) {
-    P.tile_base = s.tile_begin;
-    const uint32_t in_cts = band->num_in > s.in_base ? std::min(band->num_in - s.in_base, kSegCts) : 0u;
-    P.in = in_cts ? d_in + (size_t)s.in_base * kCtWords : nullptr;
-    P.in_bytes = in_cts * kCtWords * 4u;
-    P.out = d_out + (size_t)s.out_base * kCtWords;
-    P.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
-    const dim3 grid((s.tile_end - s.tile_begin) * wg_per_tile), block(64 * kK1Waves);
-#define IHE_LAUNCH(RT, W, RB) hipLaunchKernelGGL((k_cloud_mfma1<RT, W, RB>), grid, block, 0, ctx->stream, P)
-#define IHE_LAUNCH_RB(RT, W) if (band->row_blocks == 2) IHE_LAUNCH(RT, W, 2); else IHE_LAUNCH(RT, W, 1)
-    if (ntw == 16) { if (rot) IHE_LAUNCH_RB(true, 16); else IHE_LAUNCH_RB(false, 16); }
-    else if (ntw == 8) { if (rot) IHE_LAUNCH_RB(true, 8); else IHE_LAUNCH_RB(false, 8); }
-    else { if (rot) IHE_LAUNCH_RB(true, 4); else IHE_LAUNCH_RB(false, 4); }
-#undef IHE_LAUNCH_RB
-#undef IHE_LAUNCH
-  }
-}
-
-void launch_mfma_ws(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in, int32_t *d_out) {
-  WsParams W;
-  W.m.tiles = band->d_mtiles;
-  W.m.kcols = band->d_kcols;
-  W.m.adata = band->d_adata;
-  W.m.rowc = band->d_rowc;
-  W.m.ns = band->layout.num_samples;
-  W.m.rs = band->layout.region_size;
-  W.nkb_max = band->max_kb;
-  W.buf_words = band->max_kb * (kWsKbWords + 512) + 64;
-  W.adata_bytes = (uint32_t)std::min<size_t>(band->adata_bytes, 0xFFFFFFF0u);
-  W.rowc_bytes = (uint32_t)std::min<size_t>(band->rowc_bytes, 0xFFFFFFF0u);
-  const uint32_t lds_bytes = 2 * W.buf_words * 4;
-  const bool rot = band->layout.num_regions > 1;
-  const void *fn = rot ? (const void *)k_cloud_mfma_ws<true> : (const void *)k_cloud_mfma_ws<false>;
-  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-  for (const Segment &s : band->segments) {
-    W.m.tile_base = s.tile_begin;
-    W.ntiles = s.tile_end - s.tile_begin;
-    const uint32_t in_cts = band->num_in > s.in_base ? std::min(band->num_in - s.in_base, kSegCts) : 0u;
-    W.m.in = in_cts ? d_in + (size_t)s.in_base * kCtWords : nullptr;
-    W.m.in_bytes = in_cts * kCtWords * 4u;
-    W.m.out = d_out + (size_t)s.out_base * kCtWords;
-    W.m.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
-    // persistent grid: a multiple of 8 workgroups (one 256-word slice each)
-    const uint32_t per_slice = std::min<uint32_t>(W.ntiles, std::max<uint32_t>(1, (uint32_t)ctx->num_cus * kWsGridPerCu / 8));
-    const dim3 grid(8 * per_slice), block(64 * (kWsCons + 1));
-    if (rot) hipLaunchKernelGGL(k_cloud_mfma_ws<true>, grid, block, lds_bytes, ctx->stream, W);
-    else hipLaunchKernelGGL(k_cloud_mfma_ws<false>, grid, block, lds_bytes, ctx->stream, W);
+    MfmaEntry E = {};
+    E.tiles = band->d_mtiles;
+    E.kcols = band->d_kcols;
+    E.adata = band->d_adata;
+    E.rowc = band->d_rowc;
+    E.ns = L.num_samples;
+    E.rs = L.region_size;
+    E.R = L.num_regions;
+    E.ct_base = s.in_base / L.num_regions;
+    const uint32_t in_cts = num_in > E.ct_base ? std::min(num_in - E.ct_base, kSegCts) : 0u;
+    E.in = in_cts ? d_in + (size_t)E.ct_base * kCtWords : nullptr;
+    E.in_bytes = in_cts * kCtWords * 4u;
+    E.out = d_out + (size_t)s.out_base * kCtWords;
+    E.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
+    E.tile_base = s.tile_begin;
+    launch_mfma(ctx->stream, band->shape, L.num_regions > 1, E, nullptr, nullptr, 0,
+                s.tile_end - s.tile_begin);
   }
 }
 
 }  // namespace
 
-int ihe_cloud_score(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in,
-                    const double *d_in_var, int32_t *d_out, double *d_out_var) {
+int ihe_cloud_score_layout(ihe_ctx *ctx, const ihe_band *band, const ihe_layout *layout,
+                           uint32_t num_in_cts, const int32_t *d_in, const double *d_in_var,
+                           int32_t *d_out, double *d_out_var) {
   if (int rc = check_ctx(ctx)) return rc;
   if (!band || band->ctx != ctx) return fail(IHE_ERR_ARG, "band does not belong to this context");
+  if (int rc = check_layout(layout)) return rc;
+  const ihe_layout &L = *layout;
+  if (band->mode != kModeMfma &&
+      (L.num_samples != band->layout.num_samples || num_in_cts != band->num_in))
+    return fail(IHE_ERR_UNSUPPORTED, "only MFMA bands run on another layout than their own");
+  // every referenced ciphertext must exist (getTLWE, eval/idash.h:162-166)
+  if (band->has_cols && band->max_bidx / L.num_regions >= num_in_cts)
+    return fail(IHE_ERR_RANGE, "the model references FeatIndex " +
+                                   std::to_string(band->max_bidx / L.num_regions) + " >= " +
+                                   std::to_string(num_in_cts));
   if (band->num_out == 0) return IHE_OK;
-  if (!d_out || (band->num_in > 0 && !d_in)) return fail(IHE_ERR_ARG, "null ciphertext buffer");
-  if (d_out_var && band->num_in > 0 && !d_in_var) return fail(IHE_ERR_ARG, "null input variance");
+  if (!d_out || (num_in_cts > 0 && !d_in)) return fail(IHE_ERR_ARG, "null ciphertext buffer");
+  if (d_out_var && num_in_cts > 0 && !d_in_var) return fail(IHE_ERR_ARG, "null input variance");
   if (int rc = make_current(ctx)) return rc;
   {
     ScopedTimer t(ctx, IHE_TIMER_CLOUD);
-    if (band->mode == kModeMfmaWs) launch_mfma_ws(ctx, band, d_in, d_out);
-    else if (band->mode == kModeMfma1) launch_mfma1(ctx, band, d_in, d_out);
-    else if (band->mode == kModeMfma) launch_mfma(ctx, band, d_in, d_out);
+    if (band->mode == kModeMfma) launch_mfma_band(ctx, band, L, num_in_cts, d_in, d_out);
     else if (band->mode == kModeStaged) launch_staged(ctx, band, d_in, d_out);
     else launch_direct(ctx, band, d_in, d_out);
   }
   IHE_HIP(hipGetLastError());
   if (d_out_var) {
     ScopedTimer t(ctx, IHE_TIMER_CLOUD_VAR);
-    const dim3 vgrid((band->num_out + 255) / 256), block(256);
-    hipLaunchKernelGGL(k_cloud_var, vgrid, block, 0, ctx->stream, band->d_var_ptr, band->d_var_ct,
-                       band->d_var_w2, d_in_var, d_out_var, band->num_out);
+    const VarEntry V = {band->d_var_ptr, band->d_var_col, band->d_var_w2, d_in_var, d_out_var,
+                        band->num_out, L.num_regions, {0, 0}};
+    launch_var(ctx->stream, V, nullptr, nullptr, 0, (band->num_out + 255) / 256);
     IHE_HIP(hipGetLastError());
   }
   return IHE_OK;
+}
+
+int ihe_cloud_score(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in,
+                    const double *d_in_var, int32_t *d_out, double *d_out_var) {
+  if (!band) return fail(IHE_ERR_ARG, "null band");
+  return ihe_cloud_score_layout(ctx, band, &band->layout, band->num_in, d_in, d_in_var, d_out,
+                                d_out_var);
 }
 
 }  // extern "C"

@@ -1,0 +1,78 @@
+// Byte-split int8 MFMA cloud kernels: the interface between the band compiler
+// (ihe_cloud.hip), the batched launcher (ihe_batch.hip) and the kernels
+// (ihe_mfma.hip).
+#pragma once
+
+#include "ihe_internal.h"
+
+namespace ihe {
+
+constexpr uint32_t kMfmaRows = 32;      // outputs per MFMA row block
+constexpr uint32_t kMfmaMaxNkb = 8;     // K-blocks (of 32 columns) per tile
+constexpr uint32_t kSegCts = (1u << 18) - 1;  // ciphertexts per segment: byte offsets < 2^31
+
+typedef int int4v __attribute__((ext_vector_type(4)));
+
+// One tile: up to 32*RB consecutive output rows over nkb*32 columns. Column
+// lists hold big indices, so one tiling serves every packing layout; the
+// kernels derive the ciphertext (c / R) and rotation ((c % R) * RS) per launch.
+struct MTileDev {
+  uint32_t kcol_off;  // first column (nkb * 32 big indices)
+  uint32_t nkb;       // K-blocks actually used (<= the launch's NKB)
+  uint32_t a_off;     // first A operand block (64 lanes x 16 bytes): [row block][kb][weight byte]
+  uint32_t row_off;   // 32 * RB row biases (bias * 2^18)
+  uint32_t slot0;     // first output slot, relative to the segment's out_base
+  uint32_t nrows;     // rows in use
+  uint32_t pad0, pad1;
+};
+
+// One cohort segment as the kernels see it. Passed by value for a single
+// cohort, or read from a device table in a batch.
+struct alignas(16) MfmaEntry {
+  const MTileDev *tiles;
+  const uint32_t *kcols;
+  const int4v *adata;
+  const uint32_t *rowc;
+  const int32_t *in;   // ciphertext ct_base of the cohort
+  int32_t *out;        // output slot out_base of the cohort
+  uint32_t in_bytes, out_bytes;
+  uint32_t ns, rs, R, ct_base;  // layout; ct_base = first ciphertext of `in`
+  uint32_t tile_base;           // first tile of the segment
+  uint32_t pad[13];
+};
+static_assert(sizeof(MfmaEntry) == 128, "entry is two 64-byte scalar loads");
+
+// Kernel family of a tiling.
+struct MfmaShape {
+  int rb = 2;   // row blocks per tile (1 or 2; 2 only with nkb == 1)
+  int nkb = 1;  // K-blocks per tile
+  int ntw = 4;  // 32-word column tiles per wave (4, 8 or 16)
+};
+
+// Enqueues the MFMA cloud kernel. Single cohort: entries == nullptr and
+// `single` describes it, with num_items = its tiles. Batch: entries[e] for
+// e < num_entries, first[e] = index of entry e's first tile in the launch
+// (first[num_entries] = num_items).
+void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const MfmaEntry &single,
+                 const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
+                 uint32_t num_items);
+
+// Output variances: var_out[o] = sum over the row's entries with c % R == 0 of
+// w2 * var_in[c / R] (int32 w*w, summed in row order). Entries are stored
+// interleaved per block of 64 outputs: entry j of output 64*b + l sits at
+// (blk_off[b] + j) * 64 + l, rows padded to the block's longest with
+// kVarPad columns, so a wave reads entry j of its 64 outputs in one load.
+constexpr uint32_t kVarPad = 0xFFFFFFFFu;
+struct VarEntry {
+  const uint32_t *blk_off;  // [ceil(num_out / 64) + 1]
+  const uint32_t *col;      // big indices
+  const int32_t *w2;
+  const double *var_in;
+  double *var_out;
+  uint32_t num_out, R;
+  uint32_t pad[2];
+};
+void launch_var(hipStream_t stream, const VarEntry &single, const VarEntry *entries,
+                const uint32_t *first, uint32_t num_entries, uint32_t num_blocks);
+
+}  // namespace ihe

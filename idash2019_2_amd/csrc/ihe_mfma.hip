@@ -1,0 +1,440 @@
+// Byte-split int8 MFMA cloud kernels (gfx950, v_mfma_i32_32x32x32_i8).
+//
+// The cloud step of cloud_compute_score (eval/idash.cpp:763-848) for int16
+// weights, as a matrix product per tile: out[m][word] = sum_k W[m][k] *
+// rho_k(CT_k)[word] (mod 2^32), where k runs over the tile's columns (big
+// indices), CT_k = ciphertext k / R rotated by (k % R) * RS (the reference
+// rotates the per-region sums instead; rotation is linear mod 2^32).
+//
+// Exact on int8 MFMA: a ciphertext word x is recoded into balanced signed
+// bytes, x = sum_b 2^(8b) d_b with d_b = byte_b(x + 0x80808080) - 128 in
+// [-128, 127], and a weight into w = w_0 + 2^8 w_1 with w_0 the sign-extended
+// low byte (w in [-32768, 32639]). Then
+//   sum_k w x = sum_{b+h<=3} 2^(8(b+h)) sum_k w_h d_b   (mod 2^32)
+// because the terms with b+h >= 4 vanish mod 2^32. A K-block of 32 columns
+// costs seven MFMAs into four int32 accumulators (shifts 0, 8, 16, 24); every
+// partial sum stays below 2^31 in magnitude, and the accumulators are
+// recombined with shifts.
+//
+// Operand layout of v_mfma_i32_32x32x32_i8: A lane l holds row l%32, bytes
+// k = 16*(l/32) + j; B lane l holds column (word) l%32 with the same k; value v
+// of the result in lane l is row 8*(v/4) + 4*(l/32) + v%4, word l%32.
+#include "ihe_mfma.h"
+
+namespace ihe {
+namespace {
+
+typedef int int16v __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x16v __attribute__((ext_vector_type(16)));
+#define IHE_CONSTANT __attribute__((address_space(4)))
+constexpr int kRsrcWord3 = 0x00020000;          // gfx9 raw buffer descriptor, dword 3
+constexpr uint32_t kDropOffset = 0x80000000u;   // >= every segment's byte size: stores dropped
+constexpr uint32_t kWaves = 4;                  // waves per workgroup
+constexpr int kStorePolicy = 2;                 // nt: streaming output stores
+
+template <typename T>
+__device__ __forceinline__ const IHE_CONSTANT T *cptr(const T *p) {
+  return (const IHE_CONSTANT T *)p;
+}
+
+struct MfmaArgs {
+  MfmaEntry single;
+  const MfmaEntry *entries;
+  const uint32_t *first;
+  uint32_t num_entries;
+};
+
+struct EntryWords {
+  u32x16v lo, hi;
+};
+
+// The workgroup's entry and tile: blocks are renumbered so that each XCD (the
+// dispatcher deals blocks round-robin over 8 XCDs) walks a contiguous run of
+// items, whose column windows overlap in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+  const uint32_t per = n / 8;
+  return b < 8 * per ? (b % 8) * per + b / 8 : b;
+}
+
+__device__ __forceinline__ MfmaEntry entry_of(const MfmaArgs &A, uint32_t item, uint32_t &tile) {
+  if (!A.entries) {
+    tile = A.single.tile_base + item;
+    return A.single;
+  }
+  const IHE_CONSTANT uint32_t *first = cptr(A.first);
+  uint32_t lo = 0, hi = A.num_entries;  // first[lo] <= item < first[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (first[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  const IHE_CONSTANT u32x16v *e = (const IHE_CONSTANT u32x16v *)(A.entries + lo);
+  const EntryWords w = {e[0], e[1]};
+  const MfmaEntry E = __builtin_bit_cast(MfmaEntry, w);
+  tile = E.tile_base + (item - first[lo]);
+  return E;
+}
+
+__device__ __forceinline__ MTileDev tile_of(const MfmaEntry &E, uint32_t tile) {
+  typedef uint32_t u32x8v __attribute__((ext_vector_type(8)));
+  return __builtin_bit_cast(MTileDev, *(const IHE_CONSTANT u32x8v *)(E.tiles + tile));
+}
+
+// Column word of big index c: ciphertext byte offset relative to the
+// segment (ct - ct_base) << 13, plus the rotation shift (c % R) * RS << 2.
+template <bool ROT>
+__device__ __forceinline__ uint32_t col_word(uint32_t c, const MfmaEntry &E) {
+  if constexpr (!ROT) {
+    return (c - E.ct_base) << 13;
+  } else {
+    const uint32_t ct = c / E.R;
+    return ((ct - E.ct_base) << 13) | (((c - ct * E.R) * E.rs) << 2);
+  }
+}
+
+// Loads word `word` (0..2047: a then b) of the lane's 16 columns; a rotated
+// column reads P[i + m] for i + m < N and -P[i + m - N] past the end.
+template <bool ROT>
+__device__ __forceinline__ void load_cols(__amdgpu_buffer_rsrc_t in_r, const uint32_t (&cw)[16],
+                                          uint32_t word, uint32_t (&x)[16]) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if constexpr (!ROT) {
+      x[j] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(in_r, cw[j] + word * 4u, 0, 0);
+    } else {
+      const uint32_t t = (word & (kN - 1)) + ((cw[j] >> 2) & (kN - 1));
+      const uint32_t v = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
+          in_r, (cw[j] & ~0x1FFFu) + ((t & (kN - 1)) << 2) + (word & kN) * 4u, 0, 0);
+      x[j] = t >= kN ? 0u - v : v;
+    }
+  }
+}
+
+// 16 words -> 4 planes of balanced signed bytes (plane b = digit b of each word).
+__device__ __forceinline__ void byte_planes(const uint32_t (&x)[16], int4v (&p)[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint32_t y[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] = (x[4 * q + i] + 0x80808080u) ^ 0x80808080u;
+    const uint32_t t0 = __builtin_amdgcn_perm(y[1], y[0], 0x05010400u);
+    const uint32_t t1 = __builtin_amdgcn_perm(y[1], y[0], 0x07030602u);
+    const uint32_t t2 = __builtin_amdgcn_perm(y[3], y[2], 0x05010400u);
+    const uint32_t t3 = __builtin_amdgcn_perm(y[3], y[2], 0x07030602u);
+    p[0][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x05040100u);
+    p[1][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x07060302u);
+    p[2][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x05040100u);
+    p[3][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x07060302u);
+  }
+}
+
+struct Acc {
+  int16v s0, s8, s16, s24;  // partial sums at shifts 0, 8, 16, 24
+};
+
+__device__ __forceinline__ void mfma_kblock(const int4v &a0, const int4v &a1, const int4v (&pl)[4],
+                                            Acc &c) {
+  c.s0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[0], c.s0, 0, 0, 0);
+  c.s8 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[1], c.s8, 0, 0, 0);
+  c.s16 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[2], c.s16, 0, 0, 0);
+  c.s24 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pl[3], c.s24, 0, 0, 0);
+  c.s8 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[0], c.s8, 0, 0, 0);
+  c.s16 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[1], c.s16, 0, 0, 0);
+  c.s24 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[2], c.s24, 0, 0, 0);
+}
+
+// Recombines, applies the b-polynomial bias (samples < NS) and hiding (words
+// >= RS), and stores 32 rows x 32 words: row m of lanes 0..31 (and m + 4 of
+// lanes 32..63) for value v, m = 8*(v/4) + v%4. Rows at or past nrows store
+// out of range. Always 16 stores.
+__device__ __forceinline__ void finish_rows(const Acc &c, const MfmaEntry &E,
+                                            __amdgpu_buffer_rsrc_t out_r, const uint32_t *bias,
+                                            uint32_t n0, uint32_t r, uint32_t h, uint32_t row0,
+                                            uint32_t nrows, bool full, uint32_t rbase) {
+  const uint32_t word = n0 + r;
+  uint32_t o[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v)  // s0 + 2^8 s8 + 2^16 s16 + 2^24 s24, Horner form
+    o[v] = ((((((uint32_t)c.s24[v] << 8) + (uint32_t)c.s16[v]) << 8) + (uint32_t)c.s8[v]) << 8) +
+           (uint32_t)c.s0[v];
+  // uniform per column tile except where NS or RS falls inside it
+  const bool b_poly = n0 >= kN;
+  if (b_poly && n0 - kN < E.ns) {
+    const bool lane_bias = word - kN < E.ns;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const u32x4v bv = *(const u32x4v *)(bias + row0 + 8 * g + 4 * h);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o[4 * g + q] += lane_bias ? bv[q] : 0u;
+    }
+  }
+  if (b_poly && n0 + 31 - kN >= E.rs) {
+    const bool keep = word - kN < E.rs;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) o[v] = keep ? o[v] : 0u;
+  }
+  const uint32_t voff = word * 4u + 4 * h * kCtWords * 4u;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const uint32_t m = row0 + 8 * (v / 4) + (v % 4);
+    __builtin_amdgcn_raw_buffer_store_b32((int)o[v], out_r,
+                                          full || m + 4 * h < nrows ? voff : kDropOffset,
+                                          (int)(rbase + m * kCtWords * 4u), kStorePolicy);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One K-block (<= 32 columns), RB row blocks per tile. A wave keeps the A
+// operands and column words in registers and walks NTW column tiles of 32
+// words; each step's B operand feeds RB row blocks, so input words read per
+// output word fall to 1/RB. Every step issues 16 loads (the next step's
+// columns; the last step reloads its own) followed by 16*RB stores, so the
+// loads are waited with the stores still in flight; the first step is peeled
+// so the loop is entered with the same pending counts as it is re-entered.
+template <bool ROT, int NTW, int RB>
+__global__ __launch_bounds__(64 * kWaves) void k_cloud_mfma1(MfmaArgs A) {
+  constexpr uint32_t wg_per_tile = kCtWords / (kWaves * NTW * 32);
+  constexpr uint32_t rows = kMfmaRows * RB;
+  __shared__ uint32_t s_bias[rows];
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  uint32_t tile;
+  const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
+  const uint32_t part = blk % wg_per_tile;
+  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t h = lane / 32, r = lane % 32;
+  const MTileDev T = tile_of(E, tile);
+  const __amdgpu_buffer_rsrc_t in_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)E.in, (short)0, (int)E.in_bytes, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t out_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)E.out, (short)0, (int)E.out_bytes, kRsrcWord3);
+  if (threadIdx.x < rows) s_bias[threadIdx.x] = E.rowc[T.row_off + threadIdx.x];
+  uint32_t cw[16];
+  {
+    const IHE_CONSTANT uint32_t *kc = cptr(E.kcols) + T.kcol_off;
+    const u32x16v cl = *(const IHE_CONSTANT u32x16v *)kc;
+    const u32x16v chh = *(const IHE_CONSTANT u32x16v *)(kc + 16);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) cw[j] = col_word<ROT>(h ? chh[j] : cl[j], E);
+  }
+  int4v a[RB][2];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+    a[rb][0] = E.adata[(size_t)(T.a_off + 2 * rb) * 64 + lane];
+    a[rb][1] = E.adata[(size_t)(T.a_off + 2 * rb + 1) * 64 + lane];
+  }
+  const uint32_t nt0 = (part * kWaves + wave) * NTW;
+  const uint32_t row_base = T.slot0 * kCtWords * 4u;
+  const bool full = T.nrows == rows;
+  __syncthreads();
+
+  auto step = [&](const uint32_t (&x)[16], uint32_t nt) {
+    int4v pl[4];
+    byte_planes(x, pl);
+    // the row offsets are formed per step, not hoisted into 16*RB live SGPRs
+    uint32_t rbase = row_base;
+    asm volatile("" : "+s"(rbase));
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      Acc c = {};
+      mfma_kblock(a[rb][0], a[rb][1], pl, c);
+      finish_rows(c, E, out_r, s_bias, (nt0 + nt) * 32, r, h, 32 * rb, T.nrows, full, rbase);
+    }
+  };
+
+  uint32_t x[16], xn[16];
+  load_cols<ROT>(in_r, cw, nt0 * 32 + r, x);
+  load_cols<ROT>(in_r, cw, (nt0 + (NTW > 1 ? 1 : 0)) * 32 + r, xn);
+  step(x, 0);
+  for (uint32_t nt = 1; nt < (uint32_t)NTW; ++nt) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = xn[j];
+    const uint32_t ntn = nt + 1 < (uint32_t)NTW ? nt + 1 : nt;
+    load_cols<ROT>(in_r, cw, (nt0 + ntn) * 32 + r, xn);
+    step(x, nt);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// NKB K-blocks (up to NKB*32 columns, wider windows), one row block per tile.
+// The tile's A operands and column words live in LDS. Within a column tile the
+// K-blocks are unrolled and each issues the next K-block's 16 loads (or the
+// next column tile's first) before consuming its own, so its wait leaves 16
+// loads, or 16 loads and 16 stores, in flight.
+template <bool ROT, int NTW, int NKB>
+__global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
+  constexpr uint32_t wg_per_tile = kCtWords / (kWaves * NTW * 32);
+  __shared__ uint32_t s_cw[NKB * 32];
+  __shared__ int4v s_a[NKB * 2 * 64];
+  __shared__ uint32_t s_bias[kMfmaRows];
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  uint32_t tile;
+  const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
+  const uint32_t part = blk % wg_per_tile;
+  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t h = lane / 32, r = lane % 32;
+  const MTileDev T = tile_of(E, tile);
+  const __amdgpu_buffer_rsrc_t in_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)E.in, (short)0, (int)E.in_bytes, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t out_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)E.out, (short)0, (int)E.out_bytes, kRsrcWord3);
+  // tiles using fewer K-blocks than NKB are padded with zero-weight blocks
+  for (uint32_t i = threadIdx.x; i < NKB * 32; i += 64 * kWaves) {
+    const uint32_t kb = i / 32;
+    const uint32_t c = E.kcols[T.kcol_off + (kb < T.nkb ? i : (T.nkb - 1) * 32)];
+    s_cw[i] = col_word<ROT>(c, E);
+  }
+  for (uint32_t i = threadIdx.x; i < NKB * 2 * 64; i += 64 * kWaves)
+    s_a[i] = i < T.nkb * 2 * 64 ? E.adata[(size_t)T.a_off * 64 + i] : int4v{0, 0, 0, 0};
+  if (threadIdx.x < kMfmaRows) s_bias[threadIdx.x] = E.rowc[T.row_off + threadIdx.x];
+  const uint32_t nt0 = (part * kWaves + wave) * NTW;
+  const uint32_t row_base = T.slot0 * kCtWords * 4u;
+  const bool full = T.nrows == kMfmaRows;
+  __syncthreads();
+
+  auto cols_of = [&](uint32_t kb, uint32_t (&cw)[16]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const u32x4v v = *(const u32x4v *)(s_cw + 32 * kb + 16 * h + 4 * q);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cw[4 * q + i] = v[i];
+    }
+  };
+
+  // One column tile. Scheduling barriers keep each K-block's loads ahead of
+  // the previous K-block's compute; the next column tile's first K-block is
+  // loaded before this tile's stores.
+  uint32_t xa[16];
+  auto column_tile = [&](uint32_t nt) {
+    Acc c = {};
+    uint32_t xnext[16];
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) {
+      uint32_t xb[16];
+      {
+        uint32_t cw[16];
+        cols_of(kb + 1 < NKB ? kb + 1 : 0, cw);
+        if (kb + 1 < NKB) {
+          load_cols<ROT>(in_r, cw, (nt0 + nt) * 32 + r, xb);
+        } else {
+          const uint32_t ntn = nt + 1 < (uint32_t)NTW ? nt + 1 : nt;
+          load_cols<ROT>(in_r, cw, (nt0 + ntn) * 32 + r, xnext);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      int4v pl[4];
+      byte_planes(xa, pl);
+      mfma_kblock(s_a[(2 * kb) * 64 + lane], s_a[(2 * kb + 1) * 64 + lane], pl, c);
+      __builtin_amdgcn_sched_barrier(0);
+      if (kb + 1 < NKB) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) xa[j] = xb[j];
+      }
+    }
+    uint32_t rbase = row_base;
+    asm volatile("" : "+s"(rbase));
+    finish_rows(c, E, out_r, s_bias, (nt0 + nt) * 32, r, h, 0, T.nrows, full, rbase);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) xa[j] = xnext[j];
+  };
+  {
+    uint32_t cw[16];
+    cols_of(0, cw);
+    load_cols<ROT>(in_r, cw, nt0 * 32 + r, xa);
+  }
+  // the first column tile is peeled so that the loop is entered, like it is
+  // re-entered, with 16 loads followed by 16 stores outstanding
+  column_tile(0);
+  for (uint32_t nt = 1; nt < (uint32_t)NTW; ++nt) column_tile(nt);
+}
+
+// ---------------------------------------------------------------------------
+// Output variances, 256 outputs per block.
+struct VarArgs {
+  VarEntry single;
+  const VarEntry *entries;
+  const uint32_t *first;
+  uint32_t num_entries;
+};
+
+__global__ __launch_bounds__(256) void k_cloud_var(VarArgs A) {
+  VarEntry E = A.single;
+  uint32_t chunk = blockIdx.x;
+  if (A.entries) {
+    uint32_t lo = 0, hi = A.num_entries;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (A.first[mid] <= chunk) lo = mid;
+      else hi = mid;
+    }
+    E = A.entries[lo];
+    chunk -= A.first[lo];
+  }
+  const uint32_t blk = chunk * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t o = blk * 64 + lane;
+  if (blk * 64 >= E.num_out) return;
+  double v = 0.0;
+  const uint32_t e1 = E.blk_off[blk + 1];
+  for (uint32_t e = E.blk_off[blk]; e < e1; ++e) {
+    const uint32_t c = E.col[(size_t)e * 64 + lane];
+    const uint32_t ct = c / E.R;
+    if (c != kVarPad && c == ct * E.R)
+      v = __dadd_rn(v, __dmul_rn((double)E.w2[(size_t)e * 64 + lane], E.var_in[ct]));
+  }
+  if (o < E.num_out) E.var_out[o] = v;
+}
+
+template <bool ROT, int NTW>
+void launch_k1(hipStream_t s, int rb, dim3 grid, const MfmaArgs &A) {
+  if (rb == 2) hipLaunchKernelGGL((k_cloud_mfma1<ROT, NTW, 2>), grid, dim3(64 * kWaves), 0, s, A);
+  else hipLaunchKernelGGL((k_cloud_mfma1<ROT, NTW, 1>), grid, dim3(64 * kWaves), 0, s, A);
+}
+
+template <bool ROT, int NTW>
+void launch_kk(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
+  const dim3 block(64 * kWaves);
+  switch (nkb) {
+    case 2: hipLaunchKernelGGL((k_cloud_mfmak<ROT, NTW, 2>), grid, block, 0, s, A); break;
+    case 3: hipLaunchKernelGGL((k_cloud_mfmak<ROT, NTW, 3>), grid, block, 0, s, A); break;
+    case 4: hipLaunchKernelGGL((k_cloud_mfmak<ROT, NTW, 4>), grid, block, 0, s, A); break;
+    case 5: hipLaunchKernelGGL((k_cloud_mfmak<ROT, NTW, 5>), grid, block, 0, s, A); break;
+    case 6: hipLaunchKernelGGL((k_cloud_mfmak<ROT, NTW, 6>), grid, block, 0, s, A); break;
+    case 7: hipLaunchKernelGGL((k_cloud_mfmak<ROT, NTW, 7>), grid, block, 0, s, A); break;
+    default: hipLaunchKernelGGL((k_cloud_mfmak<ROT, NTW, 8>), grid, block, 0, s, A); break;
+  }
+}
+
+template <bool ROT>
+void launch_rot(hipStream_t s, const MfmaShape &sh, dim3 grid, const MfmaArgs &A) {
+  if (sh.nkb == 1) {
+    if (sh.ntw == 16) launch_k1<ROT, 16>(s, sh.rb, grid, A);
+    else if (sh.ntw == 8) launch_k1<ROT, 8>(s, sh.rb, grid, A);
+    else launch_k1<ROT, 4>(s, sh.rb, grid, A);
+  } else {
+    if (sh.ntw == 16) launch_kk<ROT, 16>(s, sh.nkb, grid, A);
+    else if (sh.ntw == 8) launch_kk<ROT, 8>(s, sh.nkb, grid, A);
+    else launch_kk<ROT, 4>(s, sh.nkb, grid, A);
+  }
+}
+
+}  // namespace
+
+void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const MfmaEntry &single,
+                 const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
+                 uint32_t num_items) {
+  if (num_items == 0) return;
+  const uint32_t wg_per_tile = kCtWords / (kWaves * (uint32_t)shape.ntw * 32);
+  const MfmaArgs A = {single, entries, first, num_entries};
+  const dim3 grid(num_items * wg_per_tile);
+  if (rot) launch_rot<true>(stream, shape, grid, A);
+  else launch_rot<false>(stream, shape, grid, A);
+}
+
+void launch_var(hipStream_t stream, const VarEntry &single, const VarEntry *entries,
+                const uint32_t *first, uint32_t num_entries, uint32_t num_blocks) {
+  if (num_blocks == 0) return;
+  const VarArgs A = {single, entries, first, num_entries};
+  hipLaunchKernelGGL(k_cloud_var, dim3(num_blocks), dim3(256), 0, stream, A);
+}
+
+}  // namespace ihe

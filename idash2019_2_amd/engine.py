@@ -90,6 +90,45 @@ class Band:
             self.handle = 0
 
 
+class Batch:
+    """Many cohorts' cloud steps as one launch per kernel family (ihe_batch_*)."""
+
+    def __init__(self, engine: "Engine", cohorts):
+        self.engine = engine
+        arr = (_abi.Cohort * max(1, len(cohorts)))()
+        self._keep = []
+        for i, c in enumerate(cohorts):
+            band, ns, num_in, d_in, d_in_var, d_out, d_out_var = c
+            _, R, RS = layout_for(ns)
+            arr[i].band = band.handle
+            arr[i].layout = _abi.Layout(ns, R, RS)
+            arr[i].num_in_cts = num_in
+            arr[i].d_in = d_in.ptr
+            arr[i].d_in_var = d_in_var.ptr if d_in_var is not None else None
+            arr[i].d_out = d_out.ptr
+            arr[i].d_out_var = d_out_var.ptr if d_out_var is not None else None
+            self._keep.append(c)
+        h = ctypes.c_void_p()
+        check(lib().ihe_batch_create(engine.ctx, arr, len(cohorts), ctypes.byref(h)))
+        self.handle = h.value
+
+    def capture(self) -> None:
+        check(lib().ihe_batch_capture(self.handle))
+
+    def run(self) -> None:
+        check(lib().ihe_batch_run(self.handle))
+
+    def info(self) -> dict:
+        bi = _abi.BatchInfo()
+        check(lib().ihe_batch_get_info(self.handle, ctypes.byref(bi)))
+        return {f: getattr(bi, f) for f, _ in _abi.BatchInfo._fields_}
+
+    def free(self) -> None:
+        if self.handle:
+            check(lib().ihe_batch_destroy(self.handle))
+            self.handle = 0
+
+
 class Key:
     def __init__(self, engine: "Engine", key: np.ndarray):
         key = np.ascontiguousarray(key, np.int32)
@@ -172,6 +211,31 @@ class Engine:
                                     ctypes.byref(h)))
         return Band(self, h.value, num_in, row_ptr.size - 1, (ns, R, RS))
 
+    def band_tuned(self, num_samples: int, num_in: int, row_ptr, in_bidx, weight,
+                   d_in: DeviceArray, d_out: DeviceArray, reps: int = 3) -> tuple[Band, list]:
+        """ihe_band_create_tuned: the fastest MFMA tile shape on these buffers, and
+        the timings of every candidate as (rows per tile, K-blocks, column tiles
+        per wave, ms)."""
+        ns, R, RS = layout_for(num_samples)
+        lay = _abi.Layout(ns, R, RS)
+        row_ptr = np.ascontiguousarray(row_ptr, np.uint32)
+        in_bidx = np.ascontiguousarray(in_bidx, np.uint32)
+        weight = np.ascontiguousarray(weight, np.int32)
+        m = _abi.ModelCSR(row_ptr.size - 1, row_ptr.ctypes.data, in_bidx.ctypes.data,
+                          weight.ctypes.data)
+        res = (_abi.TuneEntry * 64)()
+        n = ctypes.c_uint32(64)
+        h = ctypes.c_void_p()
+        check(lib().ihe_band_create_tuned(self.ctx, ctypes.byref(lay), num_in, ctypes.byref(m),
+                                          d_in.ptr, d_out.ptr, reps, ctypes.byref(h), res,
+                                          ctypes.byref(n)))
+        table = [(r.rows_per_tile, r.kblocks, r.col_tiles_per_wave, r.ms) for r in res[:n.value]]
+        return Band(self, h.value, num_in, row_ptr.size - 1, (ns, R, RS)), table
+
+    def batch(self, cohorts) -> Batch:
+        """cohorts: (band, num_samples, num_in, d_in, d_in_var, d_out, d_out_var) tuples."""
+        return Batch(self, cohorts)
+
     def cloud(self, band: Band, d_in: DeviceArray, d_in_var: DeviceArray | None,
               d_out: DeviceArray, d_out_var: DeviceArray | None) -> None:
         """cloud_compute_score (eval/idash.cpp:763-848) on device buffers (async)."""
@@ -182,6 +246,19 @@ class Engine:
         check(lib().ihe_cloud_score(self.ctx, band.handle, d_in.ptr,
                                     d_in_var.ptr if d_in_var is not None else None, d_out.ptr,
                                     d_out_var.ptr if d_out_var is not None else None))
+
+    def cloud_layout(self, band: Band, num_samples: int, num_in: int, d_in: DeviceArray,
+                     d_in_var: DeviceArray | None, d_out: DeviceArray,
+                     d_out_var: DeviceArray | None) -> None:
+        """The band's model on a cohort with another sample count (MFMA bands)."""
+        ns, R, RS = layout_for(num_samples)
+        lay = _abi.Layout(ns, R, RS)
+        if d_out.nbytes < band.num_out * CT_WORDS * 4 or d_in.nbytes < num_in * CT_WORDS * 4:
+            raise ValueError("buffer too small")
+        check(lib().ihe_cloud_score_layout(self.ctx, band.handle, ctypes.byref(lay), num_in,
+                                           d_in.ptr, d_in_var.ptr if d_in_var is not None else None,
+                                           d_out.ptr,
+                                           d_out_var.ptr if d_out_var is not None else None))
 
     # -- keys, encrypt, decrypt
     def key(self, key: np.ndarray) -> Key:

@@ -125,6 +125,21 @@ int ihe_timing_read(ihe_ctx *ctx, int timer, double *total_ms, uint32_t *count);
  * Fails with IHE_ERR_RANGE when a coefficient's FeatIndex >= num_in_cts. */
 int ihe_band_create(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
                     const ihe_model_csr *model, int variant, ihe_band **out);
+/* Autotuner: compiles the model once per feasible MFMA tile shape (output rows
+ * per tile, i.e. targets per workgroup; K-blocks of 32 columns, the window
+ * length a tile covers; 32-word column tiles per wave, the coefficient chunk),
+ * times `reps` cloud launches of each on the caller's cohort buffers (d_out is
+ * overwritten) and returns the fastest band in *out. results[0 .. *num_results)
+ * receive the timings (*num_results: capacity in, count out). Needs weights
+ * within the MFMA split; IHE_ERR_UNSUPPORTED otherwise. */
+typedef struct {
+  uint32_t rows_per_tile, kblocks, col_tiles_per_wave;
+  float ms;
+} ihe_tune_entry;
+int ihe_band_create_tuned(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
+                          const ihe_model_csr *model, const int32_t *d_in, int32_t *d_out,
+                          uint32_t reps, ihe_band **out, ihe_tune_entry *results,
+                          uint32_t *num_results);
 int ihe_band_destroy(ihe_band *band);
 int ihe_band_get_info(const ihe_band *band, ihe_band_info *info);
 /* Runs the band compiler on the host only (no device, no upload) and reports
@@ -138,6 +153,43 @@ int ihe_band_plan(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model
  * (NULL skips the variance). Enqueued on the context stream. */
 int ihe_cloud_score(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in,
                     const double *d_in_var, int32_t *d_out, double *d_out_var);
+
+/* The same step for a cohort with another packing layout or ciphertext count
+ * than the band was created with: MFMA bands hold their columns as big
+ * indices, so one resident band serves cohorts of every sample count (the
+ * reference re-reads the model per run, eval/cloud.cpp:14). VALU bands accept
+ * only their own layout (IHE_ERR_UNSUPPORTED otherwise). Fails with
+ * IHE_ERR_RANGE when a coefficient's FeatIndex (bidx / R) >= num_in_cts. */
+int ihe_cloud_score_layout(ihe_ctx *ctx, const ihe_band *band, const ihe_layout *layout,
+                           uint32_t num_in_cts, const int32_t *d_in, const double *d_in_var,
+                           int32_t *d_out, double *d_out_var);
+
+/* ---------------- batched cloud step ---------------- */
+/* Many cohorts in one step, in place of one cloud_compute_score call per
+ * cohort (eval/cloud.cpp:17): each cohort has its own packing layout, input
+ * ciphertexts and resident MFMA band (bands may be shared). The work is one
+ * launch per MFMA kernel family plus one variance launch, and
+ * ihe_batch_capture records them into a hipGraph so that ihe_batch_run is a
+ * single graph launch. Results equal ihe_cloud_score_layout per cohort. */
+typedef struct ihe_batch ihe_batch;
+typedef struct {
+  const ihe_band *band;    /* MFMA band */
+  ihe_layout layout;       /* this cohort's packing */
+  uint32_t num_in_cts;
+  const int32_t *d_in;     /* [num_in_cts][2][N] */
+  const double *d_in_var;  /* [num_in_cts]; may be NULL when d_out_var is */
+  int32_t *d_out;          /* [band outputs][2][N] */
+  double *d_out_var;       /* NULL: no variances for this cohort */
+} ihe_cohort;
+typedef struct {
+  uint32_t num_cohorts, num_launches, num_tiles, captured;
+} ihe_batch_info;
+int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohorts,
+                     ihe_batch **out);
+int ihe_batch_capture(ihe_batch *batch);
+int ihe_batch_run(ihe_batch *batch); /* enqueued on the context stream */
+int ihe_batch_get_info(const ihe_batch *batch, ihe_batch_info *info);
+int ihe_batch_destroy(ihe_batch *batch);
 
 /* ---------------- keys, encrypt, decrypt ---------------- */
 /* Binary TLWE key s[0..N) (tLweKeyGen, eval/idash.cpp:429-430). */

@@ -79,8 +79,10 @@ def test_band_plan_groups_shared_windows(monkeypatch):
     assert staged["num_groups"] == info["num_groups"]
     used = np.unique(3 * c.lo[:, None] + np.arange(15)[None, :]).size
     assert used <= staged["num_cols"] < 16 * info["num_groups"] / 2
-    # MFMA: tiles of 32 consecutive outputs over the union of their windows
+    # MFMA with wide K budget: tiles of 32 consecutive outputs over the union of
+    # their windows
     monkeypatch.setenv("IHE_CLOUD_KERNEL", "mfma")
+    monkeypatch.setenv("IHE_CLOUD_SHAPE", "1,4")
     mfma = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
     assert mfma["variant"] == _abi.VARIANT_MFMA
     assert mfma["num_tiles"] == -(-c.num_outputs // 32)
@@ -88,10 +90,24 @@ def test_band_plan_groups_shared_windows(monkeypatch):
     # default: single-K-block tiles of up to 64 rows, cut short where the
     # union passes 32 columns
     monkeypatch.delenv("IHE_CLOUD_KERNEL")
+    monkeypatch.delenv("IHE_CLOUD_SHAPE")
     k1 = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
     assert k1["variant"] == _abi.VARIANT_MFMA
     assert -(-c.num_outputs // 64) <= k1["num_tiles"] <= mfma["num_tiles"]
     assert used <= k1["num_cols"] <= 32 * k1["num_tiles"]
+
+
+def test_band_plan_wide_window_picks_multi_kblock_tiles(monkeypatch):
+    c = synth.generate(200, 400, 1024, 50, seed=8)      # 150 columns per output
+    info = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
+    assert info["variant"] == _abi.VARIANT_MFMA and info["max_window"] == 150
+    # tiles of <= 32 rows over >= 5 K-blocks
+    assert info["num_tiles"] >= -(-c.num_outputs // 32)
+    monkeypatch.setenv("IHE_CLOUD_SHAPE", "1,4")       # 128 columns cannot hold a row
+    assert band_plan(c.num_samples, c.num_in_cts, *c.model_csr())["variant"] == _abi.VARIANT_VALU
+    with pytest.raises(IheError) as e:
+        band_plan(c.num_samples, c.num_in_cts, *c.model_csr(), variant=_abi.VARIANT_MFMA)
+    assert e.value.code == _abi.IHE_ERR_UNSUPPORTED
 
 
 def test_band_plan_wide_weights_select_wide_variant():

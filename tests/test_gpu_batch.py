@@ -1,0 +1,142 @@
+"""Batched cloud step (ihe_batch_*) and the kernel autotuner
+(ihe_band_create_tuned): cohorts of uneven sample counts on resident model
+bands of several neighbour counts, in one launch per kernel family and as a
+captured hipGraph; every cohort bit-exact against its own per-cohort launch
+and, on small sizes, against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from idash2019_2_amd import _abi, synth
+from idash2019_2_amd._abi import IheError
+
+pytestmark = pytest.mark.gpu
+
+
+def random_cts(rng, num_in):
+    return rng.integers(-(1 << 31), 1 << 31, (num_in, 2, 1024), dtype=np.int64).astype(np.int32)
+
+
+def make_models(tags, targets, sets, neighbours):
+    """Model sets sharing one SNP layout (positions and windows) with
+    independently drawn weights and biases."""
+    models = {}
+    for n in neighbours:
+        for s in range(sets):
+            c = synth.generate(tags, targets, 1024, n, seed=n)
+            if s > 0:
+                rng = np.random.default_rng(1000 * s + n)
+                c.weights = rng.integers(-2048, 2048, c.weights.shape).astype(np.int16)
+                c.bias = rng.integers(-2048, 2048, c.bias.shape).astype(np.int32)
+            models[(s, n)] = (c, c.model_csr())
+    return models
+
+
+@pytest.fixture(scope="module")
+def setup(engine):
+    models = make_models(60, 150, 2, (5, 10, 20, 50))
+    bands = {k: engine.band(1024, c.num_in_cts, *csr) for k, (c, csr) in models.items()}
+    yield models, bands
+    for b in bands.values():
+        b.free()
+
+
+def make_cohorts(engine, models, bands, sample_counts, with_var=True):
+    rng = np.random.default_rng(7)
+    keys = list(models)
+    cohorts, host = [], []
+    for i, ns in enumerate(sample_counts):
+        key = keys[i % len(keys)]
+        c, csr = models[key]
+        R = 1024 // ns
+        num_in = (3 * c.num_tags + R - 1) // R
+        ct = random_cts(rng, num_in)
+        var = rng.integers(1, 1 << 16, num_in) * 2.0 ** -60
+        d_in, d_var = engine.to_device(ct), engine.to_device(var)
+        d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
+        d_ov = engine.empty((c.num_outputs,), np.float64) if with_var else None
+        cohorts.append((bands[key], ns, num_in, d_in, d_var, d_out, d_ov))
+        host.append((key, ns, ct, var))
+    return cohorts, host
+
+
+def test_batch_matches_per_cohort_launches_and_oracle(engine, setup):
+    models, bands = setup
+    counts = [1024, 700, 341, 200, 128, 100, 64, 33, 16, 1024, 512, 77]
+    cohorts, host = make_cohorts(engine, models, bands, counts)
+    # per-cohort launches
+    ref = []
+    for band, ns, num_in, d_in, d_var, d_out, d_ov in cohorts:
+        engine.cloud_layout(band, ns, num_in, d_in, d_var, d_out, d_ov)
+        ref.append((d_out.numpy(), d_ov.numpy()))
+        d_out.fill_zero()
+        d_ov.fill_zero()
+    batch = engine.batch(cohorts)
+    info = batch.info()
+    assert info["num_cohorts"] == len(counts)
+    assert 2 <= info["num_launches"] <= 1 + 2 * 4 * 3   # families + the variance launch
+    batch.run()
+    for (_, _, _, _, _, d_out, d_ov), (o, v) in zip(cohorts, ref):
+        assert (d_out.numpy() == o).all() and (d_ov.numpy() == v).all()
+    # the oracle on a few cohorts (including rotated layouts)
+    for i in (0, 2, 4, 8):
+        key, ns, ct, var = host[i]
+        c, (rp, ib, w) = models[key]
+        R = 1024 // ns
+        want, wv = oracle.cloud(ct, var, rp, ib, w, ns, R, 1024 // R)
+        assert (ref[i][0] == want).all() and (ref[i][1] == wv).all(), i
+    # captured as a graph: every replay equals the eager result
+    batch.capture()
+    assert batch.info()["captured"] == 1
+    for _ in range(3):
+        for _, _, _, _, _, d_out, d_ov in cohorts:
+            d_out.fill_zero()
+            d_ov.fill_zero()
+        engine.sync()
+        batch.run()
+        engine.sync()
+        for (_, _, _, _, _, d_out, d_ov), (o, v) in zip(cohorts, ref):
+            assert (d_out.numpy() == o).all() and (d_ov.numpy() == v).all()
+    batch.free()
+
+
+def test_batch_without_variances_and_errors(engine, setup):
+    models, bands = setup
+    cohorts, _ = make_cohorts(engine, models, bands, [256, 1024, 50], with_var=False)
+    batch = engine.batch(cohorts)
+    batch.run()
+    engine.sync()
+    for band, ns, num_in, d_in, d_var, d_out, _ in cohorts:
+        got = d_out.numpy()
+        engine.cloud_layout(band, ns, num_in, d_in, None, d_out, None)
+        assert (d_out.numpy() == got).all()
+    batch.free()
+    band, ns, num_in, d_in, d_var, d_out, d_ov = cohorts[0]
+    with pytest.raises(IheError) as e:
+        engine.batch([(band, ns, num_in - 1, d_in, d_var, d_out, d_ov)])
+    assert e.value.code == _abi.IHE_ERR_RANGE
+    c, csr = models[(0, 5)]
+    valu = engine.band(1024, c.num_in_cts, *csr, variant=_abi.VARIANT_VALU)
+    with pytest.raises(IheError) as e:
+        engine.batch([(valu, 1024, c.num_in_cts, d_in, d_var, d_out, d_ov)])
+    assert e.value.code == _abi.IHE_ERR_UNSUPPORTED
+    valu.free()
+
+
+@pytest.mark.parametrize("n", [5, 20])
+def test_tuned_band_matches_default(engine, n):
+    c = synth.generate(80, 300, 1024, n, seed=n)
+    rp, ib, w = c.model_csr()
+    rng = np.random.default_rng(n)
+    ct = random_cts(rng, c.num_in_cts)
+    var = np.full(c.num_in_cts, 2.0 ** -50)
+    d_in, d_var = engine.to_device(ct), engine.to_device(var)
+    d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
+    d_ov = engine.empty((c.num_outputs,), np.float64)
+    band, table = engine.band_tuned(1024, c.num_in_cts, rp, ib, w, d_in, d_out, reps=2)
+    assert len(table) >= 3 and all(t[3] > 0 for t in table)
+    assert {t[2] for t in table} == {4, 8, 16}
+    engine.cloud(band, d_in, d_var, d_out, d_ov)
+    want, wv = oracle.cloud(ct, var, rp, ib, w, 1024, 1, 1024)
+    assert (d_out.numpy() == want).all() and (d_ov.numpy() == wv).all()
+    band.free()

@@ -9,23 +9,32 @@ from idash2019_2_amd._abi import IheError
 
 pytestmark = pytest.mark.gpu
 
-# The cloud kernels: single-K-block byte-split MFMA (the default for int16
-# weights where every 32-row tile fits 32 columns), general byte-split MFMA,
-# warp-specialized LDS-staged MFMA, LDS-staged dot2 VALU and direct dot2 VALU.
-KERNELS = ["mfma1", "mfma", "ws", "staged", "direct"]
+# The cloud kernels. MFMA shapes (IHE_CLOUD_SHAPE = row blocks, K-blocks):
+# 64-row single-K-block tiles (the default at 5 neighbours), 32-row tiles,
+# and 2 or 3 K-blocks (forcing padded K-blocks on narrow windows); NTW = 32-word
+# column tiles per wave. Then the LDS-staged dot2 VALU and direct dot2 VALU.
+KERNELS = {
+    "mfma": {},
+    "mfma-rb1": {"IHE_CLOUD_SHAPE": "1,1"},
+    "mfma-k2": {"IHE_CLOUD_SHAPE": "1,2", "IHE_CLOUD_NTW": "8"},
+    "mfma-k3": {"IHE_CLOUD_SHAPE": "1,3"},
+    "mfma-ntw16": {"IHE_CLOUD_NTW": "16"},
+    "staged": {"IHE_CLOUD_KERNEL": "staged"},
+    "direct": {"IHE_CLOUD_KERNEL": "direct"},
+}
 
 
-@pytest.fixture(params=KERNELS)
+@pytest.fixture(params=list(KERNELS))
 def kernel(request, monkeypatch):
-    if request.param == "mfma1":
-        monkeypatch.delenv("IHE_CLOUD_KERNEL", raising=False)
-    else:
-        monkeypatch.setenv("IHE_CLOUD_KERNEL", request.param)
+    for k in ("IHE_CLOUD_KERNEL", "IHE_CLOUD_SHAPE", "IHE_CLOUD_NTW"):
+        monkeypatch.delenv(k, raising=False)
+    for k, v in KERNELS[request.param].items():
+        monkeypatch.setenv(k, v)
     return request.param
 
 
 def expected_variant(kernel):
-    return _abi.VARIANT_MFMA if kernel in ("mfma1", "mfma", "ws") else _abi.VARIANT_VALU
+    return _abi.VARIANT_MFMA if kernel.startswith("mfma") else _abi.VARIANT_VALU
 
 
 def run_cloud(engine, ct, var, row_ptr, in_bidx, weight, ns, variant=_abi.VARIANT_AUTO):
@@ -125,6 +134,8 @@ def test_wide_window_matches_oracle(engine, kernel):
     rp, ib, w = c.model_csr()
     out, ov, info = run_cloud(engine, ct, var, rp, ib, w, 1024)
     assert info["max_window"] == 150
+    if kernel in ("mfma", "mfma-ntw16"):
+        assert info["variant"] == _abi.VARIANT_MFMA
     ref, rv = oracle.cloud(ct, var, rp, ib, w, *c.layout)
     assert (out == ref).all() and (ov == rv).all()
 
@@ -149,7 +160,7 @@ def test_output_without_variance_buffer(engine):
     band.free()
 
 
-@pytest.mark.parametrize("kernel", ["mfma1", "mfma", "staged"], indirect=True)
+@pytest.mark.parametrize("kernel", ["mfma", "mfma-rb1", "staged"], indirect=True)
 def test_full_size_rows_match_oracle(engine, kernel):
     """The full 16,184-tag / 80,882-target / 1,024-sample workload on the GPU;
     a random subset of 600 output rows (plus the first and last) is recomputed
@@ -171,3 +182,29 @@ def test_full_size_rows_match_oracle(engine, kernel):
     assert (ov[rows] == rv).all()
     w2 = (c.weights.astype(np.int64) ** 2).sum(axis=1).astype(np.float64) * 2.0 ** -50
     assert (ov == w2).all()
+
+
+@pytest.mark.parametrize("ns", [1024, 128, 341, 16, 1])
+def test_one_band_serves_every_layout(engine, ns):
+    """An MFMA band compiled once (its columns are big indices) runs cohorts of
+    any sample count: ciphertext c / R, rotation (c % R) * RS per launch."""
+    c = synth.generate(40, 120, 1024, 5, seed=3)
+    rp, ib, w = c.model_csr()
+    band = engine.band(1024, c.num_in_cts, rp, ib, w)
+    assert band.info()["variant"] == _abi.VARIANT_MFMA
+    R = 1024 // ns
+    num_in = (3 * c.num_tags + R - 1) // R
+    rng = np.random.default_rng(ns)
+    ct = random_cts(rng, num_in)
+    var = rng.integers(1, 1 << 20, num_in) * 2.0 ** -60
+    d_in, d_var = engine.to_device(ct), engine.to_device(var)
+    d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
+    d_ov = engine.empty((c.num_outputs,), np.float64)
+    engine.cloud_layout(band, ns, num_in, d_in, d_var, d_out, d_ov)
+    ref, rv = oracle.cloud(ct, var, rp, ib, w, ns, R, 1024 // R)
+    assert (d_out.numpy() == ref).all()
+    assert (d_ov.numpy() == rv).all()
+    with pytest.raises(IheError) as e:          # a ciphertext the model needs is missing
+        engine.cloud_layout(band, ns, num_in - 1, d_in, d_var, d_out, d_ov)
+    assert e.value.code == _abi.IHE_ERR_RANGE
+    band.free()
