@@ -6,6 +6,7 @@ ABI in include/idash_hip.h); this package is its Python front end. The
 idash-compatible C++ host (keygen/encrypt/cloud/decrypt) lives in host/.
 """
 from . import _abi
-from .engine import Band, Batch, DeviceArray, Engine, Key, band_plan, layout_for, noise_table
+from .engine import (Band, Batch, DeviceArray, Engine, Key, PinnedArray, Server, band_plan,
+                     layout_for, noise_table)
 
-__all__ = ["Engine", "Band", "Batch", "DeviceArray", "Key", "band_plan", "layout_for", "noise_table", "_abi"]
+__all__ = ["Engine", "Band", "Batch", "Server", "PinnedArray", "DeviceArray", "Key", "band_plan", "layout_for", "noise_table", "_abi"]

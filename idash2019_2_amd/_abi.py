@@ -55,6 +55,20 @@ class TuneEntry(ctypes.Structure):
                 ("ms", ctypes.c_float)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", _u64), ("num_slots", _u32), ("queue_capacity", _u32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("model", _u32), ("num_samples", _u32), ("geno", _vp), ("key", _vp),
+                ("seed", _u64), ("out", _vp)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", _u64), ("pending", _u32), ("running", _u32),
+                ("arena_bytes", _u64), ("arena_used", _u64), ("arena_peak", _u64)]
+
+
 class BandInfo(ctypes.Structure):
     _fields_ = [("num_outputs", _u32), ("num_groups", _u32), ("num_tiles", _u32),
                 ("num_cols", _u32), ("nnz", _u32), ("variant", _u32), ("max_window", _u32),
@@ -99,6 +113,13 @@ SIGNATURES = {
     "ihe_batch_run": (_int, [_vp]),
     "ihe_batch_get_info": (_int, [_vp, ctypes.POINTER(BatchInfo)]),
     "ihe_batch_destroy": (_int, [_vp]),
+    "ihe_server_create": (_int, [_vp, ctypes.POINTER(ServerConfig), _pvp]),
+    "ihe_server_add_model": (_int, [_vp, _vp, _u32, ctypes.POINTER(_u32)]),
+    "ihe_server_submit": (_int, [_vp, ctypes.POINTER(Request), ctypes.POINTER(_u64)]),
+    "ihe_server_poll": (_int, [_vp, _u64, ctypes.POINTER(_int), ctypes.POINTER(ctypes.c_double)]),
+    "ihe_server_wait": (_int, [_vp, _u64, ctypes.POINTER(ctypes.c_double)]),
+    "ihe_server_get_stats": (_int, [_vp, ctypes.POINTER(ServerStats)]),
+    "ihe_server_destroy": (_int, [_vp]),
     "ihe_key_create": (_int, [_vp, _vp, _pvp]),
     "ihe_key_destroy": (_int, [_vp]),
     "ihe_encrypt": (_int, [_vp, _vp, ctypes.POINTER(Layout), _u32, _vp, _u64, _vp, _vp]),

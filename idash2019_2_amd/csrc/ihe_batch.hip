@@ -78,7 +78,8 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
     const ihe_cohort &c = cohorts[i];
     const std::string which = "cohort " + std::to_string(i) + ": ";
     const ihe_band *band = c.band;
-    if (!band || band->ctx != ctx) return fail(IHE_ERR_ARG, which + "band does not belong to this context");
+    if (!band || band->ctx->device != ctx->device)
+      return fail(IHE_ERR_ARG, which + "band is not on this context's device");
     if (band->mode != kModeMfma)
       return fail(IHE_ERR_UNSUPPORTED, which + "batches run MFMA bands only");
     const ihe_layout &L = c.layout;

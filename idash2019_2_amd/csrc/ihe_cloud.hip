@@ -1372,7 +1372,8 @@ int ihe_cloud_score_layout(ihe_ctx *ctx, const ihe_band *band, const ihe_layout 
                            uint32_t num_in_cts, const int32_t *d_in, const double *d_in_var,
                            int32_t *d_out, double *d_out_var) {
   if (int rc = check_ctx(ctx)) return rc;
-  if (!band || band->ctx != ctx) return fail(IHE_ERR_ARG, "band does not belong to this context");
+  if (!band || band->ctx->device != ctx->device)
+    return fail(IHE_ERR_ARG, "band is not on this context's device");
   if (int rc = check_layout(layout)) return rc;
   const ihe_layout &L = *layout;
   if (band->mode != kModeMfma &&

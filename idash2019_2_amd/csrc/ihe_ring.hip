@@ -229,7 +229,8 @@ int ihe_key_destroy(ihe_key *key) {
 int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint32_t num_tags,
                 const int8_t *d_geno, uint64_t seed, int32_t *d_out, double *d_out_var) {
   if (int rc = check_ctx(ctx)) return rc;
-  if (!key || key->ctx != ctx) return fail(IHE_ERR_ARG, "key does not belong to this context");
+  if (!key || key->ctx->device != ctx->device)
+    return fail(IHE_ERR_ARG, "key is not on this context's device");
   if (!layout) return fail(IHE_ERR_ARG, "null layout");
   const uint32_t NS = layout->num_samples;
   if (NS == 0 || NS > kN || layout->num_regions != kN / NS ||
@@ -263,7 +264,8 @@ int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint
 int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const int32_t *d_in,
                 uint32_t num_cts, float *d_out) {
   if (int rc = check_ctx(ctx)) return rc;
-  if (!key || key->ctx != ctx) return fail(IHE_ERR_ARG, "key does not belong to this context");
+  if (!key || key->ctx->device != ctx->device)
+    return fail(IHE_ERR_ARG, "key is not on this context's device");
   if (num_samples == 0 || num_samples > kN) return fail(IHE_ERR_ARG, "NUM_SAMPLES must be in [1, 1024]");
   if (num_cts == 0) return IHE_OK;
   if (!d_in || !d_out) return fail(IHE_ERR_ARG, "null buffer");

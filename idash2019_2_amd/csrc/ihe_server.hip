@@ -1,0 +1,387 @@
+// Serving loop: encrypt -> cloud -> decrypt requests against resident model
+// bands, with continuous admission by HBM budget.
+//
+// A request is one cohort: genotypes [num_tags][NS] of a model's tag SNPs, a
+// binary key and a seed, and a host buffer for the decrypted scores
+// [num_outputs][NS]. Requests above 1024 samples run as chunks of at most
+// 1024 samples (one packing layout each). A worker thread owns the device
+// side: every chunk takes one slot (a context with its own stream, default
+// priority) and one block of a device arena sized by the HBM budget, and runs
+// its whole pipeline there (2-D H2D of its genotype columns, encrypt, cloud on
+// the resident band, decrypt, 2-D D2H into its score columns). Chunks are
+// admitted in arrival order as soon as a slot and arena space are free, so
+// requests that arrive while others run join the device at once and their
+// stages overlap across streams. A chunk's result depends only on its request
+// (seed + chunk index selects the random streams), so it is the same alone
+// and in the mix.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "ihe_band.h"
+
+using namespace ihe;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+constexpr uint32_t kChunkSamples = 1024;
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// First-fit allocator over one device allocation (offsets, coalescing frees).
+struct Arena {
+  size_t size = 0;
+  std::map<size_t, size_t> free;  // offset -> length
+  size_t used = 0, peak = 0;
+  void init(size_t n) {
+    size = n;
+    free.clear();
+    if (n) free[0] = n;
+  }
+  bool alloc(size_t n, size_t &off) {
+    for (auto it = free.begin(); it != free.end(); ++it) {
+      if (it->second < n) continue;
+      off = it->first;
+      const size_t rest = it->second - n;
+      free.erase(it);
+      if (rest) free[off + n] = rest;
+      used += n;
+      peak = std::max(peak, used);
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    used -= n;
+    auto next = free.lower_bound(off);
+    if (next != free.end() && off + n == next->first) {
+      n += next->second;
+      next = free.erase(next);
+    }
+    if (next != free.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += n;
+        return;
+      }
+    }
+    free[off] = n;
+  }
+};
+
+struct ModelRec {
+  const ihe_band *band;
+  uint32_t num_tags, num_out;
+};
+
+struct Request {
+  ihe_request r{};
+  uint32_t chunks = 0, issued = 0, finished = 0;
+  int rc = IHE_OK;
+  std::string err;
+  Clock::time_point t_submit, t_done;
+  bool done = false;
+};
+
+struct Slot {
+  ihe_ctx *ctx = nullptr;
+  hipEvent_t ev = nullptr;
+  bool busy = false;
+  uint64_t req = 0;
+  size_t off = 0, bytes = 0;
+  ihe_key *key = nullptr;
+};
+
+// Device bytes of one chunk: genotypes, input ciphertexts and variances,
+// output ciphertexts, decrypted scores.
+struct ChunkPlan {
+  ihe_layout L;
+  uint32_t num_in;
+  size_t o_geno, o_in, o_var, o_out, o_dec, bytes;
+};
+
+ChunkPlan plan_chunk(const ModelRec &m, uint32_t ns) {
+  ChunkPlan p;
+  const uint32_t R = kN / ns;
+  p.L = {ns, R, kN / R};
+  p.num_in = (3 * m.num_tags + R - 1) / R;
+  p.o_geno = 0;
+  p.o_in = p.o_geno + align_up((size_t)m.num_tags * ns);
+  p.o_var = p.o_in + align_up((size_t)p.num_in * kCtWords * 4);
+  p.o_out = p.o_var + align_up((size_t)p.num_in * 8);
+  p.o_dec = p.o_out + align_up((size_t)m.num_out * kCtWords * 4);
+  p.bytes = p.o_dec + align_up((size_t)m.num_out * ns * 4);
+  return p;
+}
+
+}  // namespace
+
+struct ihe_server {
+  ihe_ctx *ctx = nullptr;
+  ihe_server_config cfg{};
+  void *arena_base = nullptr;
+  Arena arena;
+  std::vector<Slot> slots;
+  std::vector<ModelRec> models;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<uint64_t> pending;  // requests with chunks left to issue, arrival order
+  std::unordered_map<uint64_t, Request> reqs;
+  uint64_t next_id = 1, completed = 0;
+  bool stop = false;
+  std::thread worker;
+
+  // Enqueues chunk `k` of request `id` on slot `s` (lock held).
+  int issue(Slot &s, uint64_t id, Request &R, uint32_t k, const ChunkPlan &p) {
+    const ModelRec &m = models[R.r.model];
+    const uint32_t col0 = k * kChunkSamples, ns = p.L.num_samples;
+    char *base = static_cast<char *>(arena_base) + s.off;
+    int8_t *d_geno = reinterpret_cast<int8_t *>(base + p.o_geno);
+    int32_t *d_in = reinterpret_cast<int32_t *>(base + p.o_in);
+    double *d_var = reinterpret_cast<double *>(base + p.o_var);
+    int32_t *d_out = reinterpret_cast<int32_t *>(base + p.o_out);
+    float *d_dec = reinterpret_cast<float *>(base + p.o_dec);
+    hipStream_t st = s.ctx->stream;
+    if (int rc = make_current(s.ctx)) return rc;
+    IHE_HIP(hipMemcpy2DAsync(d_geno, ns, R.r.geno + col0, R.r.num_samples, ns, m.num_tags,
+                             hipMemcpyHostToDevice, st));
+    if (int rc = ihe_key_create(s.ctx, R.r.key, &s.key)) return rc;
+    if (int rc = ihe_encrypt(s.ctx, s.key, &p.L, m.num_tags, d_geno, R.r.seed + k, d_in, d_var))
+      return rc;
+    if (int rc = ihe_cloud_score_layout(s.ctx, m.band, &p.L, p.num_in, d_in, nullptr, d_out, nullptr))
+      return rc;
+    if (int rc = ihe_decrypt(s.ctx, s.key, ns, d_out, m.num_out, d_dec)) return rc;
+    IHE_HIP(hipMemcpy2DAsync(R.r.out + col0, (size_t)R.r.num_samples * 4, d_dec, (size_t)ns * 4,
+                             (size_t)ns * 4, m.num_out, hipMemcpyDeviceToHost, st));
+    IHE_HIP(hipEventRecord(s.ev, st));
+    s.busy = true;
+    s.req = id;
+    (void)R;
+    return IHE_OK;
+  }
+
+  void finish_chunk(Slot &s, int rc, const std::string &err) {
+    arena.release(s.off, s.bytes);
+    if (s.key) ihe_key_destroy(s.key);
+    s.key = nullptr;
+    s.busy = false;
+    auto it = reqs.find(s.req);
+    if (it == reqs.end()) return;
+    Request &R = it->second;
+    if (rc && !R.rc) R.rc = rc, R.err = err;
+    if (++R.finished == R.chunks) {
+      R.done = true;
+      R.t_done = Clock::now();
+      ++completed;
+      cv_done.notify_all();
+    }
+  }
+
+  void fail_request(uint64_t id, int rc, const std::string &err) {
+    Request &R = reqs[id];
+    R.rc = rc, R.err = err;
+    R.finished += R.chunks - R.issued;  // never issued
+    R.issued = R.chunks;
+    if (R.finished == R.chunks) {
+      R.done = true;
+      R.t_done = Clock::now();
+      ++completed;
+      cv_done.notify_all();
+    }
+  }
+
+  void run() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      bool progress = false;
+      for (Slot &s : slots) {
+        if (!s.busy) continue;
+        const hipError_t q = hipEventQuery(s.ev);
+        if (q == hipErrorNotReady) continue;
+        finish_chunk(s, q == hipSuccess ? IHE_OK : IHE_ERR_HIP,
+                     q == hipSuccess ? "" : std::string("request failed on the device: ") + hipGetErrorString(q));
+        progress = true;
+      }
+      while (!pending.empty()) {
+        const uint64_t id = pending.front();
+        Request &R = reqs[id];
+        Slot *free_slot = nullptr;
+        for (Slot &s : slots)
+          if (!s.busy) {
+            free_slot = &s;
+            break;
+          }
+        if (!free_slot) break;
+        const uint32_t k = R.issued;
+        const uint32_t ns = std::min(kChunkSamples, R.r.num_samples - k * kChunkSamples);
+        const ChunkPlan p = plan_chunk(models[R.r.model], ns);
+        if (p.bytes > arena.size) {
+          fail_request(id, IHE_ERR_NOMEM, "a chunk of this request exceeds the HBM budget");
+          pending.pop_front();
+          progress = true;
+          continue;
+        }
+        size_t off;
+        if (!arena.alloc(p.bytes, off)) break;  // wait for memory, in arrival order
+        free_slot->off = off;
+        free_slot->bytes = p.bytes;
+        if (int rc = issue(*free_slot, id, R, k, p)) {
+          // nothing of this chunk reached the device queue after the failure
+          (void)hipStreamSynchronize(free_slot->ctx->stream);
+          free_slot->req = id;
+          R.issued++;
+          finish_chunk(*free_slot, rc, ihe_last_error());
+        } else {
+          R.issued++;
+        }
+        if (R.issued == R.chunks) pending.pop_front();
+        progress = true;
+      }
+      bool busy = false;
+      for (const Slot &s : slots) busy |= s.busy;
+      if (stop && !busy && pending.empty()) break;
+      if (!progress) cv_work.wait_for(lk, std::chrono::microseconds(busy ? 20 : 2000));
+    }
+  }
+};
+
+extern "C" {
+
+int ihe_server_create(ihe_ctx *ctx, const ihe_server_config *cfg, ihe_server **out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!cfg || !out) return fail(IHE_ERR_ARG, "null argument");
+  if (cfg->num_slots == 0 || cfg->num_slots > 64) return fail(IHE_ERR_ARG, "num_slots must be in [1, 64]");
+  if (cfg->hbm_budget == 0) return fail(IHE_ERR_ARG, "zero HBM budget");
+  *out = nullptr;
+  ihe_server *s = new ihe_server();
+  s->ctx = ctx;
+  s->cfg = *cfg;
+  if (s->cfg.queue_capacity == 0) s->cfg.queue_capacity = 1024;
+  if (int rc = ihe_malloc(ctx, align_up(cfg->hbm_budget), &s->arena_base)) {
+    delete s;
+    return rc;
+  }
+  s->arena.init(align_up(cfg->hbm_budget));
+  s->slots.resize(cfg->num_slots);
+  for (Slot &sl : s->slots) {
+    int rc = ihe_ctx_create(ctx->device, &sl.ctx);
+    if (!rc && hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess)
+      rc = fail(IHE_ERR_HIP, "event creation failed");
+    if (rc) {
+      for (Slot &o : s->slots) {
+        if (o.ev) (void)hipEventDestroy(o.ev);
+        if (o.ctx) ihe_ctx_destroy(o.ctx);
+      }
+      (void)ihe_free(ctx, s->arena_base);
+      delete s;
+      return rc;
+    }
+  }
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return IHE_OK;
+}
+
+int ihe_server_add_model(ihe_server *s, const ihe_band *band, uint32_t num_tags, uint32_t *model) {
+  if (!s || !band || !model) return fail(IHE_ERR_ARG, "null argument");
+  if (band->ctx->device != s->ctx->device) return fail(IHE_ERR_ARG, "band is not on the server's device");
+  if (band->mode != kModeMfma)
+    return fail(IHE_ERR_UNSUPPORTED, "served models need MFMA bands (any sample count)");
+  if (band->has_cols && band->max_bidx >= 3 * num_tags)
+    return fail(IHE_ERR_RANGE, "the model references tags beyond num_tags");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->models.push_back({band, num_tags, band->num_out});
+  *model = (uint32_t)s->models.size() - 1;
+  return IHE_OK;
+}
+
+int ihe_server_submit(ihe_server *s, const ihe_request *req, uint64_t *ticket) {
+  if (!s || !req || !ticket) return fail(IHE_ERR_ARG, "null argument");
+  if (req->num_samples == 0 || !req->geno || !req->key || !req->out)
+    return fail(IHE_ERR_ARG, "request needs samples, genotypes, a key and an output buffer");
+  for (uint32_t i = 0; i < kN; ++i)
+    if (req->key[i] != 0 && req->key[i] != 1) return fail(IHE_ERR_ARG, "TLWE key must be binary");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (req->model >= s->models.size()) return fail(IHE_ERR_ARG, "unknown model");
+  if (s->pending.size() >= s->cfg.queue_capacity) return fail(IHE_ERR_BUSY, "request queue is full");
+  const uint64_t id = s->next_id++;
+  Request &R = s->reqs[id];
+  R.r = *req;
+  R.chunks = (req->num_samples + kChunkSamples - 1) / kChunkSamples;
+  R.t_submit = Clock::now();
+  s->pending.push_back(id);
+  *ticket = id;
+  s->cv_work.notify_one();
+  return IHE_OK;
+}
+
+static int collect(ihe_server *s, std::unordered_map<uint64_t, Request>::iterator it,
+                   double *latency_ms) {
+  const Request R = it->second;
+  s->reqs.erase(it);
+  if (latency_ms)
+    *latency_ms = std::chrono::duration<double, std::milli>(R.t_done - R.t_submit).count();
+  if (R.rc) return fail(R.rc, R.err);
+  return IHE_OK;
+}
+
+int ihe_server_poll(ihe_server *s, uint64_t ticket, int *done, double *latency_ms) {
+  if (!s || !done) return fail(IHE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return fail(IHE_ERR_ARG, "unknown ticket");
+  *done = it->second.done ? 1 : 0;
+  if (!it->second.done) return IHE_OK;
+  return collect(s, it, latency_ms);
+}
+
+int ihe_server_wait(ihe_server *s, uint64_t ticket, double *latency_ms) {
+  if (!s) return fail(IHE_ERR_ARG, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return fail(IHE_ERR_ARG, "unknown ticket");
+  s->cv_done.wait(lk, [&] { return s->reqs.at(ticket).done; });
+  return collect(s, s->reqs.find(ticket), latency_ms);
+}
+
+int ihe_server_get_stats(ihe_server *s, ihe_server_stats *st) {
+  if (!s || !st) return fail(IHE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  st->completed = s->completed;
+  st->pending = (uint32_t)s->pending.size();
+  uint32_t busy = 0;
+  for (const Slot &sl : s->slots) busy += sl.busy ? 1u : 0u;
+  st->running = busy;
+  st->arena_bytes = s->arena.size;
+  st->arena_used = s->arena.used;
+  st->arena_peak = s->arena.peak;
+  return IHE_OK;
+}
+
+int ihe_server_destroy(ihe_server *s) {
+  if (!s) return IHE_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (Slot &sl : s->slots) {
+    if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.ctx) ihe_ctx_destroy(sl.ctx);
+  }
+  (void)ihe_free(s->ctx, s->arena_base);
+  delete s;
+  return IHE_OK;
+}
+
+}  // extern "C"

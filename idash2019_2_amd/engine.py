@@ -129,6 +129,90 @@ class Batch:
             self.handle = 0
 
 
+class PinnedArray:
+    """Page-locked host memory (ihe_host_alloc) viewed as a numpy array, so that
+    the server's 2-D copies run asynchronously."""
+
+    def __init__(self, engine: "Engine", shape, dtype):
+        self.engine = engine
+        dtype = np.dtype(dtype)
+        nbytes = max(1, int(np.prod(shape)) * dtype.itemsize)
+        p = ctypes.c_void_p()
+        check(lib().ihe_host_alloc(engine.ctx, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        buf = (ctypes.c_char * nbytes).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            check(lib().ihe_host_free(self.engine.ctx, self.ptr))
+            self.ptr = 0
+
+
+class Server:
+    """The request loop of ihe_server_* (encrypt -> cloud -> decrypt per request)."""
+
+    def __init__(self, engine: "Engine", hbm_budget: int, num_slots: int = 8,
+                 queue_capacity: int = 0):
+        self.engine = engine
+        cfg = _abi.ServerConfig(hbm_budget, num_slots, queue_capacity)
+        h = ctypes.c_void_p()
+        check(lib().ihe_server_create(engine.ctx, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h.value
+        self._live = {}
+
+    def add_model(self, band: Band, num_tags: int) -> int:
+        m = ctypes.c_uint32()
+        check(lib().ihe_server_add_model(self.handle, band.handle, num_tags, ctypes.byref(m)))
+        return m.value
+
+    def submit(self, model: int, geno: np.ndarray, key: np.ndarray, seed: int,
+               out: np.ndarray) -> int:
+        """geno [num_tags][NS] int8, key [N] int32, out [num_outputs][NS] float32; all
+        three must stay alive (and unmoved) until the request completes."""
+        if geno.dtype != np.int8 or not geno.flags.c_contiguous:
+            raise ValueError("genotypes must be C-contiguous int8")
+        if out.dtype != np.float32 or not out.flags.c_contiguous or out.shape[1] != geno.shape[1]:
+            raise ValueError("scores must be C-contiguous float32 [outputs][NS]")
+        key = np.ascontiguousarray(key, np.int32)
+        r = _abi.Request(model, geno.shape[1], geno.ctypes.data, key.ctypes.data,
+                         seed & 0xFFFFFFFFFFFFFFFF, out.ctypes.data)
+        t = ctypes.c_uint64()
+        check(lib().ihe_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)))
+        self._live[t.value] = (geno, key, out)
+        return t.value
+
+    def poll(self, ticket: int):
+        """None while running, else the latency in ms."""
+        done = ctypes.c_int()
+        ms = ctypes.c_double()
+        check(lib().ihe_server_poll(self.handle, ticket, ctypes.byref(done), ctypes.byref(ms)))
+        if not done.value:
+            return None
+        self._live.pop(ticket, None)
+        return ms.value
+
+    def wait(self, ticket: int) -> float:
+        ms = ctypes.c_double()
+        try:
+            check(lib().ihe_server_wait(self.handle, ticket, ctypes.byref(ms)))
+        finally:
+            self._live.pop(ticket, None)
+        return ms.value
+
+    def stats(self) -> dict:
+        st = _abi.ServerStats()
+        check(lib().ihe_server_get_stats(self.handle, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in _abi.ServerStats._fields_}
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().ihe_server_destroy(self.handle))
+            self.handle = 0
+            self._live.clear()
+
+
 class Key:
     def __init__(self, engine: "Engine", key: np.ndarray):
         key = np.ascontiguousarray(key, np.int32)
@@ -231,6 +315,12 @@ class Engine:
                                           ctypes.byref(n)))
         table = [(r.rows_per_tile, r.kblocks, r.col_tiles_per_wave, r.ms) for r in res[:n.value]]
         return Band(self, h.value, num_in, row_ptr.size - 1, (ns, R, RS)), table
+
+    def pinned(self, shape, dtype) -> PinnedArray:
+        return PinnedArray(self, shape, dtype)
+
+    def server(self, hbm_budget: int, num_slots: int = 8, queue_capacity: int = 0) -> Server:
+        return Server(self, hbm_budget, num_slots, queue_capacity)
 
     def batch(self, cohorts) -> Batch:
         """cohorts: (band, num_samples, num_in, d_in, d_in_var, d_out, d_out_var) tuples."""

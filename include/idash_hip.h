@@ -191,6 +191,48 @@ int ihe_batch_run(ihe_batch *batch); /* enqueued on the context stream */
 int ihe_batch_get_info(const ihe_batch *batch, ihe_batch_info *info);
 int ihe_batch_destroy(ihe_batch *batch);
 
+/* ---------------- serving ---------------- */
+/* A request loop over resident model bands: each request is one cohort's
+ * encrypt -> cloud -> decrypt (encrypt_data_ph1/ph2, cloud_compute_score,
+ * decrypt_predictions; eval/idash.cpp:615-761) run on the device, from host
+ * genotypes to host scores. Requests above 1024 samples run as chunks of at
+ * most 1024. A worker thread admits chunks in arrival order whenever one of
+ * num_slots slots (each with its own stream) and space in a device arena of
+ * hbm_budget bytes are free, so requests that arrive while others run start
+ * at once and overlap on the device. Results equal the per-request path
+ * (ihe_encrypt with seed + chunk index, ihe_cloud_score_layout, ihe_decrypt). */
+typedef struct ihe_server ihe_server;
+typedef struct {
+  uint64_t hbm_budget;     /* bytes of device memory for requests in flight */
+  uint32_t num_slots;      /* chunks on the device at once, 1..64 */
+  uint32_t queue_capacity; /* waiting requests before submit returns IHE_ERR_BUSY (0: 1024) */
+} ihe_server_config;
+typedef struct {
+  uint32_t model;       /* from ihe_server_add_model */
+  uint32_t num_samples; /* NS >= 1 */
+  const int8_t *geno;   /* host [num_tags][NS], values outside {0,1,2} missing */
+  const int32_t *key;   /* host [N] binary key */
+  uint64_t seed;        /* chunk k encrypts with seed + k */
+  float *out;           /* host [num_outputs][NS] decrypted scores */
+} ihe_request;
+typedef struct {
+  uint64_t completed;
+  uint32_t pending, running;
+  uint64_t arena_bytes, arena_used, arena_peak;
+} ihe_server_stats;
+int ihe_server_create(ihe_ctx *ctx, const ihe_server_config *cfg, ihe_server **out);
+/* band: an MFMA band over the big indices of num_tags tag SNPs. */
+int ihe_server_add_model(ihe_server *server, const ihe_band *band, uint32_t num_tags,
+                         uint32_t *model);
+/* The request's host buffers must stay valid until it completes. */
+int ihe_server_submit(ihe_server *server, const ihe_request *req, uint64_t *ticket);
+/* Completed requests report their latency (submit to scores on the host) and
+ * are forgotten; one waiter per ticket. */
+int ihe_server_poll(ihe_server *server, uint64_t ticket, int *done, double *latency_ms);
+int ihe_server_wait(ihe_server *server, uint64_t ticket, double *latency_ms);
+int ihe_server_get_stats(ihe_server *server, ihe_server_stats *stats);
+int ihe_server_destroy(ihe_server *server); /* drains the queue first */
+
 /* ---------------- keys, encrypt, decrypt ---------------- */
 /* Binary TLWE key s[0..N) (tLweKeyGen, eval/idash.cpp:429-430). */
 int ihe_key_create(ihe_ctx *ctx, const int32_t *key, ihe_key **out);
