@@ -51,6 +51,193 @@ def read_traffic(workload: str, variant: str):
     return None
 
 
+def build_bands(eng, models, keys, num_in, shapes=None):
+    """Bands of several model sets; shapes {n: (rb, nkb, ntw)} pins the tile shape
+    (IHE_CLOUD_SHAPE / IHE_CLOUD_NTW are read when a band is compiled)."""
+    bands = {}
+    for k in keys:
+        env = {}
+        if shapes and k[1] in shapes:
+            rb, nkb, ntw = shapes[k[1]]
+            env = {"IHE_CLOUD_SHAPE": f"{rb},{nkb}", "IHE_CLOUD_NTW": str(ntw)}
+        os.environ.update(env)
+        try:
+            bands[k] = eng.band(1024, num_in, *models[k].model_csr())
+        finally:
+            for kk in env:
+                os.environ.pop(kk, None)
+    return bands
+
+
+def forms_batched(eng, base, log_fn, seed=4):
+    """64 cohorts, NS ~ U[16, 1024], 3 model sets x neighbours {5, 10, 20, 50} on the
+    full SNP layout: 64 per-cohort launches vs one batch (one launch per kernel family)
+    vs its hipGraph replay; every cohort's output checksum must agree."""
+    rng = np.random.default_rng(seed)
+    models = synth.model_sets(base, (5, 10, 20, 50), 3, seed)
+    specs = [(int(rng.integers(16, 1025)), int(rng.integers(0, 3)), int(rng.choice([5, 10, 20, 50])))
+             for _ in range(64)]
+    # autotune one band per neighbour count on a full-size cohort, pin that shape for the others
+    key = eng.key(base.key)
+    d_geno = eng.to_device(base.geno)
+    d_ct = eng.empty((base.num_in_cts, 2, 1024), np.int32)
+    d_var = eng.empty((base.num_in_cts,), np.float64)
+    eng.encrypt(key, 1024, d_geno, 7, d_ct, d_var)
+    d_scratch = eng.empty((base.num_outputs, 2, 1024), np.int32)
+    shapes, tune_tables = {}, {}
+    for n in (5, 10, 20, 50):
+        b, table = eng.band_tuned(1024, base.num_in_cts, *models[(0, n)].model_csr(), d_ct,
+                                  d_scratch, reps=2)
+        best = min(table, key=lambda t: t[3])
+        shapes[n] = (best[0] // 32, best[1], best[2])
+        tune_tables[n] = table
+        b.free()
+    d_scratch.free()
+    d_ct.free()
+    d_var.free()
+    d_geno.free()
+    log_fn(f"autotuned shapes (rows/32, K-blocks, column tiles per wave): {shapes}")
+    bands = build_bands(eng, models, sorted({(s, n) for _, s, n in specs}), base.num_in_cts, shapes)
+    cohorts, work = [], 0
+    for i, (ns, sset, n) in enumerate(specs):
+        cols = rng.choice(1024, ns, replace=False)
+        R = 1024 // ns
+        num_in = (3 * base.num_tags + R - 1) // R
+        d_g = eng.to_device(np.ascontiguousarray(base.geno[:, cols]))
+        d_in = eng.empty((num_in, 2, 1024), np.int32)
+        d_iv = eng.empty((num_in,), np.float64)
+        eng.encrypt(key, ns, d_g, 1000 + i, d_in, d_iv)
+        d_g.free()
+        d_out = eng.empty((base.num_outputs, 2, 1024), np.int32)
+        d_ov = eng.empty((base.num_outputs,), np.float64)
+        cohorts.append((bands[(sset, n)], ns, num_in, d_in, d_iv, d_out, d_ov))
+        work += base.num_targets * ns
+    eng.sync()
+
+    def sums():
+        return [(eng.checksum(c[5]), eng.checksum(c[6])) for c in cohorts]
+
+    def timed(fn, reps=3):
+        fn()
+        eng.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        eng.sync()
+        return (time.perf_counter() - t0) / reps
+
+    def sequential():
+        for band, ns, num_in, d_in, d_iv, d_out, d_ov in cohorts:
+            eng.cloud_layout(band, ns, num_in, d_in, d_iv, d_out, d_ov)
+
+    t_seq = timed(sequential)
+    ref = sums()
+    batch = eng.batch(cohorts)
+    t_batch = timed(batch.run)
+    same_batch = sums() == ref
+    launches = batch.info()["num_launches"]
+    batch.capture()
+    t_graph = timed(batch.run)
+    same_graph = sums() == ref
+    batch.free()
+    for c in cohorts:
+        for d in c[3:]:
+            d.free()
+    for b in bands.values():
+        b.free()
+    key.free()
+    if not (same_batch and same_graph):
+        raise SystemExit("batched outputs differ from the per-cohort launches")
+    out = {"cohorts": 64, "imputed_target_snps_per_step": work, "kernel_launches": launches,
+           "sequential_ms": round(t_seq * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3),
+           "graph_ms": round(t_graph * 1e3, 3),
+           "sequential_value": round(work / t_seq, 1), "batched_value": round(work / t_batch, 1),
+           "graph_value": round(work / t_graph, 1), "bit_exact": True,
+           "tile_shapes": {str(n): list(v) for n, v in shapes.items()}}
+    log_fn(f"batched form: 64 cohorts, sequential {t_seq * 1e3:.2f} ms, batch {t_batch * 1e3:.2f} ms "
+           f"({launches} launches), graph {t_graph * 1e3:.2f} ms; outputs identical")
+    return out
+
+
+def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, slots=8):
+    """Poisson arrivals of cohort requests, NS log-uniform in [1, 4096], each on one
+    of 3 resident 5-neighbour model sets; encrypt -> cloud -> decrypt on the device.
+    The served loop (slots in flight, admission by HBM budget) against the plain
+    per-request path (one request at a time) on the same arrival times; every
+    request's scores must agree."""
+    rng = np.random.default_rng(seed)
+    models = synth.model_sets(base, (5,), 3, seed)
+    bands = build_bands(eng, models, sorted(models), base.num_in_cts)
+    ns_list = np.exp(rng.uniform(0, np.log(4096), num_requests)).astype(int).clip(1, 4096)
+    reqs = []
+    for i, ns in enumerate(ns_list.tolist()):
+        cols = rng.integers(0, base.num_samples, ns)
+        geno = np.ascontiguousarray(base.geno[:, cols])
+        key = rng.integers(0, 2, 1024).astype(np.int32)
+        reqs.append((int(rng.integers(0, 3)), geno, key, 10_000 + 7 * i))
+    work = sum(base.num_targets * int(ns) for ns in ns_list)
+
+    def digest(a):
+        """Sum and xor of the 64-bit words (no copies of multi-GB score arrays)."""
+        raw = a.reshape(-1).view(np.uint8)
+        head = raw[:raw.size // 8 * 8].view(np.uint64)
+        tail = bytes(raw[raw.size // 8 * 8:])
+        return (int(head.sum(dtype=np.uint64)), int(np.bitwise_xor.reduce(head)) if head.size else 0,
+                tail, a.shape)
+
+    def run(nslots, arrivals):
+        srv = eng.server(budget, num_slots=nslots)
+        mids = [srv.add_model(bands[(s, 5)], base.num_tags) for s in range(3)]
+        tickets, outs, lat, dig = [], {}, {}, {}
+        t0 = time.perf_counter()
+        pending = []
+        for i, (m, geno, key, seed_i) in enumerate(reqs):
+            if arrivals is not None:
+                while time.perf_counter() - t0 < arrivals[i]:
+                    time.sleep(0.0002)
+            out = np.empty((base.num_outputs, geno.shape[1]), np.float32)
+            outs[i] = out
+            pending.append((i, srv.submit(mids[m], geno, key, seed_i, out)))
+            still = []
+            for j, t in pending:
+                ms = srv.poll(t)
+                if ms is None:
+                    still.append((j, t))
+                else:
+                    lat[j] = ms
+                    dig[j] = digest(outs.pop(j))
+            pending = still
+        for j, t in pending:
+            lat[j] = srv.wait(t)
+            dig[j] = digest(outs.pop(j))
+        elapsed = time.perf_counter() - t0
+        peak = srv.stats()["arena_peak"]
+        srv.close()
+        return elapsed, [lat[i] for i in range(len(reqs))], [dig[i] for i in range(len(reqs))], peak
+
+    t_plain_bb, _, dig_ref, _ = run(1, None)              # back to back: service capacity
+    rate = 0.8 * len(reqs) / t_plain_bb                     # offered load 0.8 of the plain path
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    arrivals -= arrivals[0]
+    t_srv, lat_srv, dig_srv, peak = run(slots, arrivals)
+    t_pl, lat_pl, dig_pl, _ = run(1, arrivals)
+    for b in bands.values():
+        b.free()
+    if dig_srv != dig_ref or dig_pl != dig_ref:
+        raise SystemExit("served scores differ from the per-request path")
+    p = lambda v, q: round(float(np.percentile(v, q)), 2)
+    out = {"requests": len(reqs), "arrival_rate_per_s": round(rate, 2), "slots": slots,
+           "hbm_budget_gib": budget >> 30, "arena_peak_gib": round(peak / (1 << 30), 2),
+           "served_value": round(work / t_srv, 1), "served_p50_ms": p(lat_srv, 50),
+           "served_p99_ms": p(lat_srv, 99), "plain_value": round(work / t_pl, 1),
+           "plain_p50_ms": p(lat_pl, 50), "plain_p99_ms": p(lat_pl, 99),
+           "imputed_target_snps": work, "bit_exact": True}
+    log_fn(f"served form: {len(reqs)} requests at {rate:.1f}/s: served p50 {out['served_p50_ms']} ms "
+           f"p99 {out['served_p99_ms']} ms, plain p50 {out['plain_p50_ms']} ms p99 "
+           f"{out['plain_p99_ms']} ms; scores identical")
+    return out
+
+
 def cpu_baseline(eng, d_ct, c, rp, ib, w, d_out, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -72,6 +259,9 @@ def main(argv=None):
     ap.add_argument("--variant", default="auto", choices=sorted(VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--forms", action="store_true",
+                    help="also measure the batched form (64 cohorts) and the served form "
+                         "(64 Poisson requests) on the full SNP layout")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (a kernel of known "
                          "traffic for scaling HBM counters) and exit")
@@ -173,6 +363,9 @@ def main(argv=None):
                              f"x {ns} samples), compute only, as eval/cloud.cpp:16-18 times it"}
         band.free()
         key.free()
+        forms = None
+        if args.forms:
+            forms = {"batched": forms_batched(eng, c, log), "served": forms_served(eng, c, log)}
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "imputed target SNPs/s",
@@ -191,6 +384,8 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if forms is not None:
+        line["forms"] = forms
     print(json.dumps(line), flush=True)
     return 0
 

@@ -93,6 +93,7 @@ SIGNATURES = {
     "ihe_memcpy_d2h_async": (_int, [_vp, _vp, _vp, _sz]),
     "ihe_memset": (_int, [_vp, _vp, _int, _sz]),
     "ihe_memcpy_d2d": (_int, [_vp, _vp, _vp, _sz]),
+    "ihe_checksum": (_int, [_vp, _vp, _sz, ctypes.POINTER(_u64)]),
     "ihe_timing_enable": (_int, [_vp, _int]),
     "ihe_timing_reset": (_int, [_vp]),
     "ihe_timing_read": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double),

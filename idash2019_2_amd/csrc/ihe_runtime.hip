@@ -3,6 +3,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <vector>
+
 #include "ihe_internal.h"
 
 namespace ihe {
@@ -246,9 +248,59 @@ __global__ __launch_bounds__(256) void k_copy16(const u32x4 *__restrict__ src, u
     __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+// Checksum partials: word i of the buffer contributes splitmix64((i << 32) ^ w_i),
+// summed mod 2^64 per block; the host adds the block partials.
+constexpr uint32_t kSumBlocks = 1024;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 27;
+  x *= 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_checksum(const uint32_t *__restrict__ p, size_t n,
+                                                  uint64_t *__restrict__ partial) {
+  __shared__ uint64_t red[256];
+  uint64_t acc = 0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    acc += mix64(((uint64_t)i << 32) ^ p[i]);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 }  // namespace ihe
 
 extern "C" {
+
+int ihe_checksum(ihe_ctx *ctx, const void *d_ptr, size_t bytes, uint64_t *out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!out || (bytes && !d_ptr) || bytes % 4) return fail(IHE_ERR_ARG, "bad checksum arguments");
+  *out = 0;
+  if (bytes == 0) return IHE_OK;
+  IHE_HIP(hipSetDevice(ctx->device));
+  uint64_t *d_part = nullptr;
+  IHE_HIP(hipMalloc(&d_part, kSumBlocks * sizeof(uint64_t)));
+  hipLaunchKernelGGL(k_checksum, dim3(kSumBlocks), dim3(256), 0, ctx->stream,
+                     (const uint32_t *)d_ptr, bytes / 4, d_part);
+  std::vector<uint64_t> h(kSumBlocks);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), d_part, kSumBlocks * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_part);
+  if (e != hipSuccess) return fail(IHE_ERR_HIP, std::string("checksum: ") + hipGetErrorString(e));
+  for (uint64_t v : h) *out += v;
+  return IHE_OK;
+}
 
 int ihe_memcpy_d2d(ihe_ctx *ctx, void *dst, const void *src, size_t bytes) {
   if (int rc = check_ctx(ctx)) return rc;

@@ -263,6 +263,13 @@ class Engine:
     def sync(self) -> None:
         check(lib().ihe_sync(self.ctx))
 
+    def checksum(self, d: DeviceArray, nbytes: int | None = None) -> int:
+        """ihe_checksum of the first nbytes (default: all) of a device array."""
+        out = ctypes.c_uint64()
+        check(lib().ihe_checksum(self.ctx, d.ptr, d.nbytes if nbytes is None else nbytes,
+                                 ctypes.byref(out)))
+        return out.value
+
     def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int | None = None) -> None:
         n = min(dst.nbytes, src.nbytes) if nbytes is None else nbytes
         check(lib().ihe_memcpy_d2d(self.ctx, dst.ptr, src.ptr, n))

@@ -118,6 +118,19 @@ def _windows(tags: np.ndarray, targets: np.ndarray, n: int) -> np.ndarray:
     return lo.astype(np.int64)
 
 
+def _weights(r_w, T: int, n: int):
+    """Per (target, class, tag, value) weights and per-output biases, scaled so
+    that |bias| + sum over tags of max_v |w| <= 8191, stored as int16/int32."""
+    w = r_w.integers(-2048, 2048, size=(T, 3, n, 3), dtype=np.int64)
+    b = r_w.integers(-2048, 2048, size=(T, 3), dtype=np.int64)
+    B = np.abs(b) + np.abs(w).max(axis=3).sum(axis=2)          # [T][3]
+    scale = B > 8191
+    Bs = np.where(scale, B, 1)
+    w = np.where(scale[:, :, None, None], np.sign(w) * ((np.abs(w) * 8191) // Bs[:, :, None, None]), w)
+    b = np.where(scale, np.sign(b) * ((np.abs(b) * 8191) // Bs), b)
+    return w.reshape(3 * T, 3 * n).astype(np.int16), b.reshape(3 * T).astype(np.int32)
+
+
 def generate(num_tags: int, num_targets: int, num_samples: int, neighbours: int,
              seed: int) -> Cohort:
     ss = np.random.SeedSequence(seed)
@@ -156,16 +169,7 @@ def generate(num_tags: int, num_targets: int, num_samples: int, neighbours: int,
 
     lo = _windows(tag_pos, target_pos, neighbours)
     n = min(neighbours, tag_pos.size)
-    T = target_pos.size
-    w = r_w.integers(-2048, 2048, size=(T, 3, n, 3), dtype=np.int64)
-    b = r_w.integers(-2048, 2048, size=(T, 3), dtype=np.int64)
-    B = np.abs(b) + np.abs(w).max(axis=3).sum(axis=2)          # [T][3]
-    scale = B > 8191
-    Bs = np.where(scale, B, 1)
-    w = np.where(scale[:, :, None, None], np.sign(w) * ((np.abs(w) * 8191) // Bs[:, :, None, None]), w)
-    b = np.where(scale, np.sign(b) * ((np.abs(b) * 8191) // Bs), b)
-    weights = w.reshape(3 * T, 3 * n).astype(np.int16)
-    bias = b.reshape(3 * T).astype(np.int32)
+    weights, bias = _weights(r_w, target_pos.size, n)
     key = r_key.integers(0, 2, size=N, dtype=np.int32)
     return Cohort(num_samples, n, tag_pos.astype(np.uint64), target_pos.astype(np.uint64), geno,
                   lo, weights, bias, key, seed)
@@ -222,3 +226,18 @@ def write_reference_files(c: Cohort, directory: str, nan_fraction: float = 0.0,
                 f.write("".join(lines))
     return {"tags": tags, "targets": targets, "model": os.path.join(directory, "model"),
             "geno": geno}
+
+
+def model_sets(base: Cohort, neighbours, sets: int, seed: int) -> dict:
+    """Model sets sharing base's SNP layout (positions, genotypes): for every
+    neighbour count a window per target, and for every set independently drawn
+    weights. Returns {(set, n): Cohort} (the batched and served workloads)."""
+    out = {}
+    for n in neighbours:
+        lo = _windows(base.tag_pos, base.target_pos, n)
+        for s in range(sets):
+            r_w = np.random.default_rng(np.random.SeedSequence([seed, n, s]))
+            weights, bias = _weights(r_w, base.num_targets, n)
+            out[(s, n)] = Cohort(base.num_samples, n, base.tag_pos, base.target_pos, base.geno,
+                                 lo, weights, bias, base.key, seed)
+    return out

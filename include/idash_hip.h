@@ -103,6 +103,11 @@ int ihe_memset(ihe_ctx *ctx, void *dst, int value, size_t bytes);
  * that HBM traffic counters are calibrated against. */
 int ihe_memcpy_d2d(ihe_ctx *ctx, void *dst, const void *src, size_t bytes);
 
+/* Order-sensitive 64-bit checksum of a device buffer (bytes % 4 == 0):
+ * sum over words i of splitmix64((i << 32) ^ w_i) mod 2^64. Compares large
+ * outputs of two runs without copying them to the host. Synchronizes. */
+int ihe_checksum(ihe_ctx *ctx, const void *d_ptr, size_t bytes, uint64_t *out);
+
 /* ---------------- timing of the library's own kernels ---------------- */
 enum {
   IHE_TIMER_CLOUD = 0,     /* main cloud kernel */
