@@ -27,7 +27,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from idash2019_2_amd import Engine, _abi, synth  # noqa: E402
+from idash2019_2_amd import Engine, Server, _abi, synth  # noqa: E402
 
 METRIC = ("encrypted target SNPs imputed/s at 1 GPU; % HBM roofline; "
           "p50/p99 cohort latency")
@@ -172,9 +172,11 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
     reqs = []
     for i, ns in enumerate(ns_list.tolist()):
         cols = rng.integers(0, base.num_samples, ns)
-        geno = np.ascontiguousarray(base.geno[:, cols])
+        geno = Server.to_chunks(base.geno[:, cols]) if ns > 1024 else \
+            np.ascontiguousarray(base.geno[:, cols])
+        geno_ns = ns
         key = rng.integers(0, 2, 1024).astype(np.int32)
-        reqs.append((int(rng.integers(0, 3)), geno, key, 10_000 + 7 * i))
+        reqs.append((int(rng.integers(0, 3)), geno, key, 10_000 + 7 * i, geno_ns))
     work = sum(base.num_targets * int(ns) for ns in ns_list)
 
     def digest(a):
@@ -191,13 +193,13 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
         tickets, outs, lat, dig = [], {}, {}, {}
         t0 = time.perf_counter()
         pending = []
-        for i, (m, geno, key, seed_i) in enumerate(reqs):
+        for i, (m, geno, key, seed_i, ns_i) in enumerate(reqs):
             if arrivals is not None:
                 while time.perf_counter() - t0 < arrivals[i]:
                     time.sleep(0.0002)
-            out = np.empty((base.num_outputs, geno.shape[1]), np.float32)
+            out = np.empty(base.num_outputs * ns_i, np.float32)
             outs[i] = out
-            pending.append((i, srv.submit(mids[m], geno, key, seed_i, out)))
+            pending.append((i, srv.submit(mids[m], geno, key, seed_i, out, num_samples=ns_i)))
             still = []
             for j, t in pending:
                 ms = srv.poll(t)
@@ -210,12 +212,16 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
         for j, t in pending:
             lat[j] = srv.wait(t)
             dig[j] = digest(outs.pop(j))
+            if len(dig) % 16 == 0:
+                log_fn(f"served: {len(dig)} of {len(reqs)} requests done")
         elapsed = time.perf_counter() - t0
         peak = srv.stats()["arena_peak"]
         srv.close()
         return elapsed, [lat[i] for i in range(len(reqs))], [dig[i] for i in range(len(reqs))], peak
 
+    log_fn(f"served form: {len(reqs)} requests, {sum(int(x) for x in ns_list)} samples in all")
     t_plain_bb, _, dig_ref, _ = run(1, None)              # back to back: service capacity
+    log_fn(f"plain path back to back: {t_plain_bb:.2f} s")
     rate = 0.8 * len(reqs) / t_plain_bb                     # offered load 0.8 of the plain path
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
     arrivals -= arrivals[0]

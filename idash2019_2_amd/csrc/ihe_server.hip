@@ -1,14 +1,16 @@
 // Serving loop: encrypt -> cloud -> decrypt requests against resident model
 // bands, with continuous admission by HBM budget.
 //
-// A request is one cohort: genotypes [num_tags][NS] of a model's tag SNPs, a
-// binary key and a seed, and a host buffer for the decrypted scores
-// [num_outputs][NS]. Requests above 1024 samples run as chunks of at most
-// 1024 samples (one packing layout each). A worker thread owns the device
-// side: every chunk takes one slot (a context with its own stream, default
-// priority) and one block of a device arena sized by the HBM budget, and runs
-// its whole pipeline there (2-D H2D of its genotype columns, encrypt, cloud on
-// the resident band, decrypt, 2-D D2H into its score columns). Chunks are
+// A request is one cohort: genotypes of a model's tag SNPs for NS samples, a
+// binary key and a seed, and a host buffer for the decrypted scores of every
+// output and sample. Requests above 1024 samples run as chunks of at most
+// 1024 samples (one packing layout each); the host buffers are chunk-major
+// (chunk k: genotypes [num_tags][ns_k], scores [num_outputs][ns_k]), so each
+// chunk moves with one contiguous copy each way. A worker thread owns the
+// device side: every chunk takes one slot (a context with its own stream,
+// default priority) and one block of a device arena sized by the HBM budget,
+// and runs its whole pipeline there (H2D of its genotypes, encrypt, cloud on
+// the resident band, decrypt, D2H of its scores). Chunks are
 // admitted in arrival order as soon as a slot and arena space are free, so
 // requests that arrive while others run join the device at once and their
 // stages overlap across streams. A chunk's result depends only on its request
@@ -152,16 +154,16 @@ struct ihe_server {
     float *d_dec = reinterpret_cast<float *>(base + p.o_dec);
     hipStream_t st = s.ctx->stream;
     if (int rc = make_current(s.ctx)) return rc;
-    IHE_HIP(hipMemcpy2DAsync(d_geno, ns, R.r.geno + col0, R.r.num_samples, ns, m.num_tags,
-                             hipMemcpyHostToDevice, st));
+    IHE_HIP(hipMemcpyAsync(d_geno, R.r.geno + (size_t)m.num_tags * col0, (size_t)m.num_tags * ns,
+                           hipMemcpyHostToDevice, st));
     if (int rc = ihe_key_create(s.ctx, R.r.key, &s.key)) return rc;
     if (int rc = ihe_encrypt(s.ctx, s.key, &p.L, m.num_tags, d_geno, R.r.seed + k, d_in, d_var))
       return rc;
     if (int rc = ihe_cloud_score_layout(s.ctx, m.band, &p.L, p.num_in, d_in, nullptr, d_out, nullptr))
       return rc;
     if (int rc = ihe_decrypt(s.ctx, s.key, ns, d_out, m.num_out, d_dec)) return rc;
-    IHE_HIP(hipMemcpy2DAsync(R.r.out + col0, (size_t)R.r.num_samples * 4, d_dec, (size_t)ns * 4,
-                             (size_t)ns * 4, m.num_out, hipMemcpyDeviceToHost, st));
+    IHE_HIP(hipMemcpyAsync(R.r.out + (size_t)m.num_out * col0, d_dec, (size_t)m.num_out * ns * 4,
+                           hipMemcpyDeviceToHost, st));
     IHE_HIP(hipEventRecord(s.ev, st));
     s.busy = true;
     s.req = id;

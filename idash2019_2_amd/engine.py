@@ -167,16 +167,36 @@ class Server:
         check(lib().ihe_server_add_model(self.handle, band.handle, num_tags, ctypes.byref(m)))
         return m.value
 
+    @staticmethod
+    def to_chunks(a: np.ndarray) -> np.ndarray:
+        """[rows][NS] -> the chunk-major layout of requests (1-D): chunk k holds
+        columns 1024k .. 1024k + 1023 as a contiguous [rows][ns_k] block."""
+        return np.concatenate([np.ascontiguousarray(a[:, k:k + 1024]).ravel()
+                               for k in range(0, a.shape[1], 1024)])
+
+    @staticmethod
+    def from_chunks(flat: np.ndarray, rows: int, ns: int) -> np.ndarray:
+        """Inverse of to_chunks."""
+        out = np.empty((rows, ns), flat.dtype)
+        off = 0
+        for k in range(0, ns, 1024):
+            w = min(1024, ns - k)
+            out[:, k:k + w] = flat[off:off + rows * w].reshape(rows, w)
+            off += rows * w
+        return out
+
     def submit(self, model: int, geno: np.ndarray, key: np.ndarray, seed: int,
-               out: np.ndarray) -> int:
-        """geno [num_tags][NS] int8, key [N] int32, out [num_outputs][NS] float32; all
-        three must stay alive (and unmoved) until the request completes."""
+               out: np.ndarray, num_samples: int | None = None) -> int:
+        """geno int8 and out float32 in the chunk-major request layout (for NS <= 1024
+        simply [num_tags][NS] and [num_outputs][NS]; see to_chunks), key [N] int32;
+        all three must stay alive (and unmoved) until the request completes."""
+        ns = num_samples if num_samples is not None else geno.shape[-1]
         if geno.dtype != np.int8 or not geno.flags.c_contiguous:
             raise ValueError("genotypes must be C-contiguous int8")
-        if out.dtype != np.float32 or not out.flags.c_contiguous or out.shape[1] != geno.shape[1]:
-            raise ValueError("scores must be C-contiguous float32 [outputs][NS]")
+        if out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError("scores must be C-contiguous float32")
         key = np.ascontiguousarray(key, np.int32)
-        r = _abi.Request(model, geno.shape[1], geno.ctypes.data, key.ctypes.data,
+        r = _abi.Request(model, ns, geno.ctypes.data, key.ctypes.data,
                          seed & 0xFFFFFFFFFFFFFFFF, out.ctypes.data)
         t = ctypes.c_uint64()
         check(lib().ihe_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)))

@@ -201,7 +201,10 @@ int ihe_batch_destroy(ihe_batch *batch);
  * encrypt -> cloud -> decrypt (encrypt_data_ph1/ph2, cloud_compute_score,
  * decrypt_predictions; eval/idash.cpp:615-761) run on the device, from host
  * genotypes to host scores. Requests above 1024 samples run as chunks of at
- * most 1024. A worker thread admits chunks in arrival order whenever one of
+ * most 1024, and the request's host buffers are chunk-major: chunk k (samples
+ * 1024k .. 1024k + ns_k - 1) is a contiguous [num_tags][ns_k] block of
+ * genotypes and a contiguous [num_outputs][ns_k] block of scores (for
+ * NS <= 1024 simply [num_tags][NS] and [num_outputs][NS]). A worker thread admits chunks in arrival order whenever one of
  * num_slots slots (each with its own stream) and space in a device arena of
  * hbm_budget bytes are free, so requests that arrive while others run start
  * at once and overlap on the device. Results equal the per-request path
@@ -215,10 +218,10 @@ typedef struct {
 typedef struct {
   uint32_t model;       /* from ihe_server_add_model */
   uint32_t num_samples; /* NS >= 1 */
-  const int8_t *geno;   /* host [num_tags][NS], values outside {0,1,2} missing */
+  const int8_t *geno;   /* host, chunk-major [num_tags][ns_k] blocks; outside {0,1,2}: missing */
   const int32_t *key;   /* host [N] binary key */
   uint64_t seed;        /* chunk k encrypts with seed + k */
-  float *out;           /* host [num_outputs][NS] decrypted scores */
+  float *out;           /* host, chunk-major [num_outputs][ns_k] blocks of decrypted scores */
 } ihe_request;
 typedef struct {
   uint64_t completed;

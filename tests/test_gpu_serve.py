@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from idash2019_2_amd import _abi, synth
+from idash2019_2_amd import Server, _abi, synth
 from idash2019_2_amd._abi import IheError
 
 pytestmark = pytest.mark.gpu
@@ -70,9 +70,10 @@ def serve(engine, models, reqs, slots, budget, one_at_a_time):
     outs, lat = [], []
     tickets = []
     for n, geno, key, seed in reqs:
-        out = np.full((models[n][0].num_outputs, geno.shape[1]), np.nan, np.float32)
-        outs.append(out)
-        t = srv.submit(ids[n], geno, key, seed, out)
+        rows, ns = models[n][0].num_outputs, geno.shape[1]
+        out = np.full(rows * ns, np.nan, np.float32)
+        outs.append((out, rows, ns))
+        t = srv.submit(ids[n], Server.to_chunks(geno), key, seed, out, num_samples=ns)
         if one_at_a_time:
             lat.append(srv.wait(t))
         else:
@@ -81,7 +82,7 @@ def serve(engine, models, reqs, slots, budget, one_at_a_time):
         lat.append(srv.wait(t))
     st = srv.stats()
     srv.close()
-    return outs, lat, st
+    return [Server.from_chunks(o, rows, ns) for o, rows, ns in outs], lat, st
 
 
 def test_results_same_alone_in_mix_and_per_request(engine, models):
