@@ -10,6 +10,7 @@
 // slides a J-register window down A2 as k advances, so each key bit costs one
 // LDS read per lane plus, when the bit is set, J adds. The key bits are
 // kernel arguments, hence wave-uniform branches.
+#include <cstdlib>
 #include <cstring>
 
 #include "ihe_internal.h"
@@ -89,6 +90,114 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_decrypt(const int32_t *
       // t32tod then float: the int32 -> float conversion rounds to nearest
       // even; scaling by 2^-32 is exact.
       out[(size_t)ct * ns + j] = (float)phase * 0x1p-32f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Decryption on int8 MFMA. For every ciphertext c and sample j < NS,
+//   (a_c * s)[j] = sum_i a_c[i] T[i][j],  T[i][j] = S[j - i],
+// with S[m] = s[m] for m >= 0 and -s[m + N] for m < 0 (the negacyclic wrap),
+// so decryption is the product of the ciphertexts' a-rows with one Toeplitz
+// matrix of entries in {-1, 0, 1}. Each a word is recoded into balanced
+// signed bytes (x = sum_p 2^(8p) d_p, d_p = byte_p(x + 0x80808080) - 128), one
+// v_mfma_i32_32x32x32_i8 per byte plane and K-block of 32 coefficients, four
+// int32 accumulators recombined with shifts mod 2^32 (|partial| <= 2^17).
+//
+// A workgroup of 8 waves owns 32 ciphertexts x 256 samples (one 32-sample
+// column tile per wave). Per K-block the workgroup loads the 32 x 32 a-words
+// (two per lane, four K-blocks ahead of use), writes their byte planes to LDS
+// (double-buffered), and every wave reads the full A operand back; a wave's B
+// operand (its 32 columns, the K-block's 32 rows of T) is 16 consecutive bytes
+// of u[x] = S[-x] in LDS, gathered with byte funnel shifts. Workgroups of one
+// ciphertext group run on one XCD.
+constexpr uint32_t kDecWaves = 8;
+constexpr uint32_t kDecCols = 32 * kDecWaves;  // samples per workgroup
+constexpr uint32_t kDecAhead = 4;              // K-blocks of a-words in flight
+
+struct DecryptParams {
+  const int32_t *__restrict__ in;
+  float *__restrict__ out;
+  uint32_t num_cts, ns, col_groups;
+};
+
+__global__ __launch_bounds__(64 * kDecWaves) void k_decrypt_mfma(DecryptParams P, KeyBits key) {
+  typedef int int4v __attribute__((ext_vector_type(4)));
+  typedef int int16v __attribute__((ext_vector_type(16)));
+  typedef uint32_t u32x2v __attribute__((ext_vector_type(2)));
+  __shared__ __attribute__((aligned(16))) int8_t u[2 * kN + 64];
+  __shared__ __attribute__((aligned(16))) uint16_t planes[2][4][32][16];
+  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t h = lane / 32, c = lane % 32;
+  // XCD-aware order: consecutive logical blocks (the column groups of one
+  // ciphertext group) share an XCD and its L2
+  const uint32_t n = gridDim.x, per = n / 8;
+  const uint32_t blk = blockIdx.x < 8 * per ? (blockIdx.x % 8) * per + blockIdx.x / 8 : blockIdx.x;
+  const uint32_t ct0 = (blk / P.col_groups) * 32;
+  const uint32_t j0 = (blk % P.col_groups) * kDecCols + wave * 32;
+  // u[N + x] = S[-x], x in (-N, N); zero padding around
+  for (uint32_t t = threadIdx.x; t < 2 * kN + 64; t += 64 * kDecWaves) {
+    const int32_t m = (int32_t)kN - (int32_t)t;  // S index
+    int8_t v = 0;
+    if (m >= 0 && m < (int32_t)kN) v = (int8_t)((key.w[m / 32] >> (m % 32)) & 1u);
+    else if (m < 0 && m > -(int32_t)kN) v = (int8_t)-(int8_t)((key.w[(m + kN) / 32] >> ((m + kN) % 32)) & 1u);
+    u[t] = v;
+  }
+  // this thread's share of each K-block: ciphertext ct0 + (tid % 32), words
+  // 2 * (tid / 32) .. +1 of the block's 32 coefficients
+  const uint32_t row = threadIdx.x % 32, pair = threadIdx.x / 32;
+  const uint32_t my_ct = min(ct0 + row, P.num_cts - 1);
+  const int32_t *arow = P.in + (size_t)my_ct * kCtWords + 2 * pair;
+  const bool active = j0 < P.ns;
+  int16v acc[4] = {};
+  u32x2v x[kDecAhead];
+#pragma unroll
+  for (uint32_t q = 0; q < kDecAhead; ++q) x[q] = *(const u32x2v *)(arow + 32 * q);
+  for (uint32_t kb = 0; kb < kN / 32; kb += kDecAhead) {
+#pragma unroll
+    for (uint32_t q = 0; q < kDecAhead; ++q) {
+      const uint32_t ks = kb + q, buf = ks & 1;
+      {
+        const uint32_t y0 = (x[q][0] + 0x80808080u) ^ 0x80808080u;
+        const uint32_t y1 = (x[q][1] + 0x80808080u) ^ 0x80808080u;
+        const uint32_t t0 = __builtin_amdgcn_perm(y1, y0, 0x05010400u);  // planes 0, 1
+        const uint32_t t1 = __builtin_amdgcn_perm(y1, y0, 0x07030602u);  // planes 2, 3
+        planes[buf][0][row][pair] = (uint16_t)t0;
+        planes[buf][1][row][pair] = (uint16_t)(t0 >> 16);
+        planes[buf][2][row][pair] = (uint16_t)t1;
+        planes[buf][3][row][pair] = (uint16_t)(t1 >> 16);
+      }
+      if (ks + kDecAhead < kN / 32) x[q] = *(const u32x2v *)(arow + 32 * (ks + kDecAhead));
+      __syncthreads();
+      if (active) {
+        // B: T rows 32*ks + 16*h + jj, column j0 + c: u[N - D - c + 16h + jj],
+        // D = j0 - 32*ks (start in [1, 2N - 16])
+        const uint32_t start = kN - j0 + 32 * ks - c + 16 * h;
+        const uint32_t base = start & ~3u, sh = start & 3u;
+        uint32_t w[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) w[i] = *(const uint32_t *)(u + base + 4 * i);
+        int4v b;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i] = (int)__builtin_amdgcn_alignbyte(w[i + 1], w[i], sh);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const int4v a = *(const int4v *)&planes[buf][p][c][8 * h];
+          acc[p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc[p], 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (!active) return;
+  const uint32_t j = j0 + c;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const uint32_t ct = ct0 + 8 * (v / 4) + 4 * h + (v % 4);
+    const uint32_t as = ((((((uint32_t)acc[3][v] << 8) + (uint32_t)acc[2][v]) << 8) +
+                          (uint32_t)acc[1][v]) << 8) + (uint32_t)acc[0][v];
+    if (ct < P.num_cts && j < P.ns) {
+      const int32_t phase = (int32_t)((uint32_t)P.in[(size_t)ct * kCtWords + kN + j] - as);
+      P.out[(size_t)ct * P.ns + j] = (float)phase * 0x1p-32f;
     }
   }
 }
@@ -270,8 +379,16 @@ int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const in
   if (num_cts == 0) return IHE_OK;
   if (!d_in || !d_out) return fail(IHE_ERR_ARG, "null buffer");
   if (int rc = make_current(ctx)) return rc;
-  const dim3 grid((num_cts + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   ScopedTimer t(ctx, IHE_TIMER_DECRYPT);
+  const char *force = std::getenv("IHE_DECRYPT_KERNEL");
+  if (!(force && std::strcmp(force, "valu") == 0)) {
+    DecryptParams P{d_in, d_out, num_cts, num_samples, (num_samples + kDecCols - 1) / kDecCols};
+    const dim3 grid(((num_cts + 31) / 32) * P.col_groups);
+    hipLaunchKernelGGL(k_decrypt_mfma, grid, dim3(64 * kDecWaves), 0, ctx->stream, P, key->bits);
+    IHE_HIP(hipGetLastError());
+    return IHE_OK;
+  }
+  const dim3 grid((num_cts + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   // J = outputs per lane, the smallest power of two with 64 * J >= NS.
   if (num_samples <= 64) hipLaunchKernelGGL(k_decrypt<1>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);
   else if (num_samples <= 128) hipLaunchKernelGGL(k_decrypt<2>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);

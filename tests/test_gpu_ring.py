@@ -32,20 +32,32 @@ def test_encrypt_matches_oracle(engine, ns):
     k.free()
 
 
+# decrypt kernels: the int8-MFMA Toeplitz product (default) and the VALU
+# sliding-window kernel (IHE_DECRYPT_KERNEL=valu)
+@pytest.fixture(params=["mfma", "valu"])
+def dec_kernel(request, monkeypatch):
+    if request.param == "valu":
+        monkeypatch.setenv("IHE_DECRYPT_KERNEL", "valu")
+    else:
+        monkeypatch.delenv("IHE_DECRYPT_KERNEL", raising=False)
+    return request.param
+
+
 @pytest.mark.parametrize("ns", [1, 16, 64, 65, 128, 200, 256, 512, 513, 1000, 1024])
-def test_decrypt_matches_oracle(engine, ns):
+def test_decrypt_matches_oracle(engine, dec_kernel, ns):
     rng = np.random.default_rng(ns)
     key = rng.integers(0, 2, 1024).astype(np.int32)
-    cts = rng.integers(-(1 << 31), 1 << 31, (9, 2, 1024)).astype(np.int32)
+    n = 75                       # two full groups of 32 ciphertexts and a partial one
+    cts = rng.integers(-(1 << 31), 1 << 31, (n, 2, 1024)).astype(np.int32)
     k = engine.key(key)
     d_in = engine.to_device(cts)
-    d_out = engine.empty((9, ns), np.float32)
-    engine.decrypt(k, ns, d_in, 9, d_out)
+    d_out = engine.empty((n, ns), np.float32)
+    engine.decrypt(k, ns, d_in, n, d_out)
     assert (d_out.numpy() == oracle.decrypt(cts, key, ns)).all()
     k.free()
 
 
-def test_decrypt_extreme_keys(engine):
+def test_decrypt_extreme_keys(engine, dec_kernel):
     for key in (np.zeros(1024, np.int32), np.ones(1024, np.int32)):
         rng = np.random.default_rng(int(key[0]))
         cts = rng.integers(-(1 << 31), 1 << 31, (5, 2, 1024)).astype(np.int32)
