@@ -178,6 +178,12 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
         key = rng.integers(0, 2, 1024).astype(np.int32)
         reqs.append((int(rng.integers(0, 3)), geno, key, 10_000 + 7 * i, geno_ns))
     work = sum(base.num_targets * int(ns) for ns in ns_list)
+    # page-locked score buffers (allocated once, reused by every run) so that the
+    # device-to-host copies of the scores run asynchronously
+    t_pin = time.perf_counter()
+    pinned = [eng.pinned((base.num_outputs * int(ns),), np.float32) for ns in ns_list]
+    log_fn(f"served form: {sum(p.array.nbytes for p in pinned) / 2**30:.1f} GiB of pinned score "
+           f"buffers in {time.perf_counter() - t_pin:.1f} s")
 
     def digest(a):
         """Sum and xor of the 64-bit words (no copies of multi-GB score arrays)."""
@@ -197,7 +203,7 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
             if arrivals is not None:
                 while time.perf_counter() - t0 < arrivals[i]:
                     time.sleep(0.0002)
-            out = np.empty(base.num_outputs * ns_i, np.float32)
+            out = pinned[i].array
             outs[i] = out
             pending.append((i, srv.submit(mids[m], geno, key, seed_i, out, num_samples=ns_i)))
             still = []
@@ -220,7 +226,10 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
         return elapsed, [lat[i] for i in range(len(reqs))], [dig[i] for i in range(len(reqs))], peak
 
     log_fn(f"served form: {len(reqs)} requests, {sum(int(x) for x in ns_list)} samples in all")
-    t_plain_bb, _, dig_ref, _ = run(1, None)              # back to back: service capacity
+    run(slots, None)                                       # warm-up: first DMA to the buffers
+    t_srv_bb, _, dig_bb, _ = run(slots, None)              # back to back: service capacity
+    log_fn(f"served back to back: {t_srv_bb:.2f} s")
+    t_plain_bb, _, dig_ref, _ = run(1, None)
     log_fn(f"plain path back to back: {t_plain_bb:.2f} s")
     rate = 0.8 * len(reqs) / t_plain_bb                     # offered load 0.8 of the plain path
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
@@ -229,7 +238,9 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
     t_pl, lat_pl, dig_pl, _ = run(1, arrivals)
     for b in bands.values():
         b.free()
-    if dig_srv != dig_ref or dig_pl != dig_ref:
+    for pa in pinned:
+        pa.free()
+    if dig_srv != dig_ref or dig_pl != dig_ref or dig_bb != dig_ref:
         raise SystemExit("served scores differ from the per-request path")
     p = lambda v, q: round(float(np.percentile(v, q)), 2)
     out = {"requests": len(reqs), "arrival_rate_per_s": round(rate, 2), "slots": slots,
@@ -237,6 +248,8 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
            "served_value": round(work / t_srv, 1), "served_p50_ms": p(lat_srv, 50),
            "served_p99_ms": p(lat_srv, 99), "plain_value": round(work / t_pl, 1),
            "plain_p50_ms": p(lat_pl, 50), "plain_p99_ms": p(lat_pl, 99),
+           "served_capacity_value": round(work / t_srv_bb, 1),
+           "plain_capacity_value": round(work / t_plain_bb, 1),
            "imputed_target_snps": work, "bit_exact": True}
     log_fn(f"served form: {len(reqs)} requests at {rate:.1f}/s: served p50 {out['served_p50_ms']} ms "
            f"p99 {out['served_p99_ms']} ms, plain p50 {out['plain_p50_ms']} ms p99 "
