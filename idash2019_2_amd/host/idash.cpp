@@ -7,6 +7,8 @@
 #include "idash.h"
 
 #include <sys/resource.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <sys/time.h>
 
 #include <algorithm>
@@ -319,23 +321,126 @@ std::pair<uint64_t, uint8_t> split_pos_snp(const string &s) {
 }
 }  // namespace
 
+namespace {
+
+// Model cache: <model dir>/.ihe_model_cache holds every .hr file's entries
+// (already mapped to big indices) with the file's size and modification time,
+// and a signature of the tag index; it is used when all of these still match.
+constexpr char kCacheMagic[8] = {'I', 'H', 'E', 'M', 'O', 'D', 'L', '1'};
+
+struct HrFile {
+  uint64_t pos;
+  uint32_t snp, out_bidx;
+  string path;
+  uint64_t size = 0;
+  int64_t mtime_ns = -1;
+  std::vector<std::pair<FeatBigIndex, int32_t>> entries;  // in read() order
+};
+
+uint64_t index_signature(const IdashParams &p) {
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ p.NUM_INPUT_POSITIONS;
+  for (const auto &e : p.in_features_index) {  // order-independent mix
+    uint64_t x = e.first * 0xBF58476D1CE4E5B9ull ^ ((uint64_t)e.second[0] << 32 | e.second[2]);
+    x ^= x >> 31;
+    h += x * 0x94D049BB133111EBull;
+  }
+  return h;
+}
+
+bool stat_file(HrFile &f) {
+  struct stat st;
+  if (::stat(f.path.c_str(), &st)) return false;
+  f.size = (uint64_t)st.st_size;
+  f.mtime_ns = (int64_t)st.st_mtim.tv_sec * 1000000000ll + st.st_mtim.tv_nsec;
+  return true;
+}
+
+bool load_cache(const string &file, uint64_t sig, std::vector<HrFile> &files) {
+  FILE *f = std::fopen(file.c_str(), "rb");
+  if (!f) return false;
+  char magic[8];
+  uint64_t csig = 0, n = 0;
+  bool ok = std::fread(magic, 8, 1, f) == 1 && std::memcmp(magic, kCacheMagic, 8) == 0 &&
+            std::fread(&csig, 8, 1, f) == 1 && csig == sig && std::fread(&n, 8, 1, f) == 1 &&
+            n == files.size();
+  for (size_t i = 0; ok && i < files.size(); ++i) {
+    uint64_t pos, size, cnt;
+    int64_t mt;
+    uint32_t snp, bidx;
+    ok = std::fread(&pos, 8, 1, f) == 1 && std::fread(&snp, 4, 1, f) == 1 &&
+         std::fread(&bidx, 4, 1, f) == 1 && std::fread(&size, 8, 1, f) == 1 &&
+         std::fread(&mt, 8, 1, f) == 1 && std::fread(&cnt, 8, 1, f) == 1 && pos == files[i].pos &&
+         snp == files[i].snp && bidx == files[i].out_bidx && size == files[i].size &&
+         mt == files[i].mtime_ns && cnt < (1u << 24);
+    if (ok) {
+      files[i].entries.resize(cnt);
+      ok = cnt == 0 || std::fread(files[i].entries.data(), sizeof(files[i].entries[0]), cnt, f) == cnt;
+    }
+  }
+  std::fclose(f);
+  return ok;
+}
+
+void save_cache(const string &file, uint64_t sig, const std::vector<HrFile> &files) {
+  const string tmp = file + ".tmp" + std::to_string(::getpid());
+  FILE *f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return;  // read-only model directory: no cache
+  const uint64_t n = files.size();
+  bool ok = std::fwrite(kCacheMagic, 8, 1, f) == 1 && std::fwrite(&sig, 8, 1, f) == 1 &&
+            std::fwrite(&n, 8, 1, f) == 1;
+  for (const HrFile &h : files) {
+    const uint64_t cnt = h.entries.size();
+    ok = ok && std::fwrite(&h.pos, 8, 1, f) == 1 && std::fwrite(&h.snp, 4, 1, f) == 1 &&
+         std::fwrite(&h.out_bidx, 4, 1, f) == 1 && std::fwrite(&h.size, 8, 1, f) == 1 &&
+         std::fwrite(&h.mtime_ns, 8, 1, f) == 1 && std::fwrite(&cnt, 8, 1, f) == 1 &&
+         (cnt == 0 || std::fwrite(h.entries.data(), sizeof(h.entries[0]), cnt, f) == cnt);
+  }
+  ok = std::fclose(f) == 0 && ok;
+  if (ok) ok = std::rename(tmp.c_str(), file.c_str()) == 0;
+  if (!ok) std::remove(tmp.c_str());
+}
+
+}  // namespace
+
+// One .hr file per target position and class (eval/idash.cpp:66-90), read
+// with read() (eval/parse_vw.cpp:7-29) and mapped to big indices; the files
+// are parsed in parallel, and a cache in the model directory replaces the
+// parsing while every file keeps its size and modification time.
 void read_model(Model &model, const IdashParams &params, const string &path) {
-  for (const auto &e : params.out_features_index) {
-    const uint64_t pos = e.first;
-    for (int snp = 0; snp < 3; ++snp) {
-      const string file = path + "/" + std::to_string(pos) + "_" + std::to_string(snp) + ".hr";
-      const std::unordered_map<string, int32_t> coefs = read(file);
-      std::unordered_map<FeatBigIndex, int32_t> &m = model.model[e.second[snp]] =
-          std::unordered_map<FeatBigIndex, int32_t>();
-      for (const auto &c : coefs) {
+  std::vector<HrFile> files;
+  files.reserve(3 * params.out_features_index.size());
+  for (const auto &e : params.out_features_index)
+    for (uint32_t snp = 0; snp < 3; ++snp)
+      files.push_back({e.first, snp, e.second[snp],
+                       path + "/" + std::to_string(e.first) + "_" + std::to_string(snp) + ".hr"});
+  const int64_t n = (int64_t)files.size();
+  std::vector<char> present(n);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; ++i) present[i] = stat_file(files[i]);
+  const uint64_t sig = index_signature(params);
+  const string cache = path + "/.ihe_model_cache";
+  bool all_present = true;
+  for (char c : present) all_present = all_present && c;
+  if (!(all_present && load_cache(cache, sig, files))) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+      HrFile &h = files[i];
+      h.entries.clear();
+      for (const auto &c : read(h.path)) {  // aborts on a missing file, like the reference
         if (c.first == "Constant") {
-          m[params.constant_bigIndex()] = c.second;
+          h.entries.emplace_back(params.constant_bigIndex(), c.second);
         } else {
           const auto ps = split_pos_snp(c.first);
-          m[params.inBigIdx(ps.first, ps.second)] = c.second;
+          h.entries.emplace_back(params.inBigIdx(ps.first, ps.second), c.second);
         }
       }
     }
+    save_cache(cache, sig, files);
+  }
+  for (const HrFile &h : files) {
+    std::unordered_map<FeatBigIndex, int32_t> &m = model.model[h.out_bidx] =
+        std::unordered_map<FeatBigIndex, int32_t>();
+    for (const auto &c : h.entries) m[c.first] = c.second;
   }
 }
 
@@ -569,34 +674,36 @@ void write_decrypted_predictions(const DecryptedPredictions &predictions, const 
                                  const string &filename, const bool PRINT_POS_NAME) {
   FILE *f = std::fopen(filename.c_str(), "wb");
   REQUIRE_DRAMATICALLY(f, "Cannot open result file for write");
-  std::vector<char> buf(16u << 20);
-  size_t used = 0;
-  auto emit = [&](const char *s, size_t n) {
-    if (used + n > buf.size()) {
-      std::fwrite(buf.data(), 1, used, f);
-      used = 0;
-    }
-    std::memcpy(buf.data() + used, s, n);
-    used += n;
-  };
   const char header[] = "Subject ID,target SNP,0,1,2\n";
-  emit(header, sizeof(header) - 1);
+  std::fwrite(header, 1, sizeof(header) - 1, f);
   std::vector<const std::array<std::vector<float>, 3> *> rows;
+  std::vector<string> labels;
   rows.reserve(params.out_position_names.size());
-  for (const auto &e : params.out_position_names) rows.push_back(&predictions.score.at(e.first));
-  char line[256];
-  for (uint64_t s = 0; s < params.NUM_SAMPLES; ++s) {
-    for (size_t t = 0; t < rows.size(); ++t) {
-      const auto &pr = *rows[t];
-      const auto &pn = params.out_position_names[t];
-      const string label = PRINT_POS_NAME ? pn.second : std::to_string(pn.first);
-      const int n = std::snprintf(line, sizeof(line), "%llu,%s,%g,%g,%g\n", (unsigned long long)s,
-                                  label.c_str(), (double)pr[0][s], (double)pr[1][s], (double)pr[2][s]);
-      emit(line, (size_t)n);
-    }
+  for (const auto &e : params.out_position_names) {
+    rows.push_back(&predictions.score.at(e.first));
+    labels.push_back(PRINT_POS_NAME ? e.second : std::to_string(e.first));
   }
-  std::fwrite(buf.data(), 1, used, f);
-  std::fclose(f);
+  // samples are formatted in parallel, one buffer each, and written in order
+  const int64_t ns = params.NUM_SAMPLES, block = 64;
+  std::vector<string> text(block);
+  for (int64_t s0 = 0; s0 < ns; s0 += block) {
+    const int64_t s1 = std::min(ns, s0 + block);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t s = s0; s < s1; ++s) {
+      string &out = text[s - s0];
+      out.clear();
+      char line[320];
+      for (size_t t = 0; t < rows.size(); ++t) {
+        const auto &pr = *rows[t];
+        const int n = std::snprintf(line, sizeof(line), "%lld,%s,%g,%g,%g\n", (long long)s,
+                                    labels[t].c_str(), (double)pr[0][s], (double)pr[1][s],
+                                    (double)pr[2][s]);
+        out.append(line, (size_t)std::min<int>(n, (int)sizeof(line) - 1));
+      }
+    }
+    for (int64_t s = s0; s < s1; ++s) std::fwrite(text[s - s0].data(), 1, text[s - s0].size(), f);
+  }
+  REQUIRE_DRAMATICALLY(std::fclose(f) == 0, "result file write failed");
 }
 
 // ---------------------------------------------------------------------------

@@ -54,6 +54,10 @@ def test_cli_pipeline_matches_oracle(tmp_path, ns, nan):
     run("keygen", "targets.txt", "tags.txt", "1", cwd=d)
     run("encrypt", "tags.txt", cwd=d)
     run("cloud", "model", cwd=d)
+    first = open(os.path.join(d, "encrypted_prediction.bin"), "rb").read()
+    assert os.path.exists(os.path.join(d, "model", ".ihe_model_cache"))
+    run("cloud", "model", cwd=d)                 # the second run reads the model cache
+    assert open(os.path.join(d, "encrypted_prediction.bin"), "rb").read() == first
     out = run("decrypt", cwd=d)
     assert "decrypt wall time" in out
     p, key, _ = formats.read_key(os.path.join(d, "keys.bin"))

@@ -96,3 +96,53 @@ def test_missing_files_die_dramatically(tmp_path):
     assert r.returncode != 0 and "ERROR: Cannot open parameters file for read" in r.stdout
     r = run("decrypt", cwd=tmp_path)
     assert r.returncode != 0 and "ERROR: Cannot open key file for read" in r.stdout
+
+
+def read_model_cache(path):
+    import struct
+    buf = open(path, "rb").read()
+    assert buf[:8] == b"IHEMODL1"
+    _, n = struct.unpack_from("<QQ", buf, 8)
+    off, files = 24, {}
+    for _ in range(n):
+        pos, snp, bidx, size, mtime, cnt = struct.unpack_from("<QIIQqQ", buf, off)
+        off += 40
+        ent = np.frombuffer(buf, np.int32, 2 * cnt, off).reshape(cnt, 2)
+        off += 8 * cnt
+        files[(pos, snp)] = (bidx, {int(c) & 0xFFFFFFFF: int(w) for c, w in ent})
+    return files
+
+
+def test_model_cache_is_written_validated_and_refreshed(tmp_path):
+    """read_model parses the .hr files once, keeps their entries (mapped to big
+    indices) in <model dir>/.ihe_model_cache, and re-parses when a file changes."""
+    import os
+    import time
+    c = synth.generate(20, 30, 64, 3, seed=9)
+    synth.write_reference_files(c, str(tmp_path))
+    assert run("keygen", "targets.txt", "tags.txt", "1", cwd=tmp_path).returncode == 0
+    # no encrypted data yet: cloud reads params and the model, then stops
+    r = run("cloud", "model", cwd=tmp_path)
+    assert r.returncode != 0 and "Cannot open encrypted data file" in r.stdout
+    cache = tmp_path / "model" / ".ihe_model_cache"
+    files = read_model_cache(cache)
+    p = formats.read_params(str(tmp_path / "params.bin"))
+    assert len(files) == 3 * c.num_targets
+    for t, pos in enumerate(c.target_pos.tolist()):
+        for k in range(3):
+            bidx, ent = files[(pos, k)]
+            assert bidx == 3 * t + k
+            lo = int(c.lo[t])
+            want = {3 * (lo + j // 3) + j % 3: int(c.weights[3 * t + k, j]) for j in range(3 * c.neighbours)}
+            want[0xFFFFFFFF] = int(c.bias[3 * t + k])
+            assert ent == want
+    # an edited file (new content, new mtime) is parsed again
+    pos = c.target_pos.tolist()[4]
+    hr = tmp_path / "model" / f"{pos}_1.hr"
+    time.sleep(0.01)
+    hr.write_text("".join(l if not l.startswith("Constant") else "Constant 4321\n"
+                          for l in hr.read_text().splitlines(True)))
+    os.utime(hr, ns=(time.time_ns(), time.time_ns()))
+    run("cloud", "model", cwd=tmp_path)
+    assert read_model_cache(cache)[(pos, 1)][1][0xFFFFFFFF] == 4321
+    assert p["NUM_OUTPUT_FEATURES"] == 3 * c.num_targets
