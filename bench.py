@@ -257,6 +257,49 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
     return out
 
 
+def cli_pipeline(workload, log_fn, workdir=None):
+    """The idash-compatible executables on the workload's files in the reference's
+    formats: keygen, encrypt, cloud (twice: parsing the .hr files, then from the
+    model cache), decrypt; returns their own BENCHMARK figures."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    host = os.path.join(ROOT, "idash2019_2_amd", "host")
+    d = workdir or tempfile.mkdtemp(prefix="ihe_cli_")
+    t0 = time.time()
+    c = synth.workload(workload)
+    synth.write_reference_files(c, d)
+    log_fn(f"cli: wrote {workload} input files ({3 * c.num_targets} .hr files) in "
+           f"{time.time() - t0:.1f} s")
+    figures = {}
+
+    def run(name, *args):
+        t = time.time()
+        r = subprocess.run([os.path.join(host, name), *args], cwd=d, capture_output=True, text=True,
+                           env=dict(os.environ, IHE_SEED="1"))
+        if r.returncode:
+            raise SystemExit(f"{name} failed: {r.stdout[-2000:]}{r.stderr[-2000:]}")
+        vals = {k.strip(" ."): float(v) for k, v in re.findall(r"^(.*?):\s*([0-9.e+-]+)\s*$",
+                                                                   r.stdout, re.M)}
+        vals["process wall time (s)"] = round(time.time() - t, 3)
+        log_fn(f"cli: {name} {' '.join(args)}: {vals}")
+        return vals
+
+    try:
+        figures["keygen"] = run("keygen", "targets.txt", "tags.txt", "1")
+        figures["encrypt"] = run("encrypt", "tags.txt")
+        figures["cloud_parse"] = run("cloud", "model")
+        figures["cloud_cached"] = run("cloud", "model")
+        figures["decrypt"] = run("decrypt")
+        sizes = {f: os.path.getsize(os.path.join(d, f)) for f in
+                 ("encrypted_data.bin", "encrypted_prediction.bin", "result.csv")}
+    finally:
+        if workdir is None:
+            shutil.rmtree(d, ignore_errors=True)
+    return {"workload": workload, "figures": figures, "file_bytes": sizes}
+
+
 def cpu_baseline(eng, d_ct, c, rp, ib, w, d_out, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -281,6 +324,9 @@ def main(argv=None):
     ap.add_argument("--forms", action="store_true",
                     help="also measure the batched form (64 cohorts) and the served form "
                          "(64 Poisson requests) on the full SNP layout")
+    ap.add_argument("--cli", action="store_true",
+                    help="only run the idash-compatible executables end to end on the workload's "
+                         "files and print their timings as JSON")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (a kernel of known "
                          "traffic for scaling HBM counters) and exit")
@@ -291,6 +337,10 @@ def main(argv=None):
               "shards by target range with no exchange, and multi-GPU runs are not part of "
               "this build", file=sys.stderr)
         return 2
+
+    if args.cli:
+        print(json.dumps(cli_pipeline(args.workload, log)), flush=True)
+        return 0
 
     if args.calibrate_copy:
         with Engine(0) as eng:
