@@ -374,8 +374,9 @@ def main(argv=None):
         d_out = eng.empty((c.num_outputs, 2, 1024), np.int32)
         d_ov = eng.empty((c.num_outputs,), np.float64)
         eng.sync()
-        log(f"band: {info['num_groups']} groups, {info['num_tiles']} tiles, variant "
-            f"{variant_name}")
+        log(f"band: {info['num_groups']} groups, {info['num_tiles']} tiles of {info['tile_rows']} "
+            f"rows x {info['tile_kblocks']} K-blocks ({info['col_tiles']} column tiles per wave or "
+            f"workgroup), variant {variant_name}")
 
         for _ in range(args.warmup):
             eng.cloud(band, d_ct, d_var, d_out, d_ov)
@@ -402,6 +403,33 @@ def main(argv=None):
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": read_traffic(args.workload, variant_name)}
+
+        # the bandwidth HBM sustains for the same read and write volumes, from a
+        # contiguous streaming kernel (the attainable ceiling, beside the spec peak)
+        in_b = min(C * 8192, (1 << 31) - 1024) // 1024 * 1024
+        out_b = min(c.num_outputs * 8192, (1 << 31) - 1024) // 65536 * 65536
+        p_in, p_out = eng.empty((in_b,), np.uint8), eng.empty((out_b,), np.uint8)
+        p_in.fill_zero()
+        for _ in range(2):
+            eng.stream_probe(p_in, in_b, p_out, out_b)
+        eng.sync()
+        eng.timing(True)
+        eng.timing_reset()
+        for _ in range(10):
+            eng.stream_probe(p_in, in_b, p_out, out_b)
+        probe_ms, probe_n = eng.timing_read(_abi.TIMER_STREAM)
+        eng.timing(False)
+        p_in.free()
+        p_out.free()
+        probe_ms /= probe_n
+        ceiling = {"probe": "contiguous 16-byte reads and writes interleaved per wave, the "
+                            "volumes one cloud step reads and writes",
+                   "read_bytes": in_b, "write_bytes": out_b, "ms": round(probe_ms, 4),
+                   "GB_per_s": round((in_b + out_b) / probe_ms / 1e6, 1),
+                   "cloud_kernel_frac": round(probe_ms / (avg_kernel_s * 1e3), 4)}
+        log(f"streaming probe over the same volumes: {probe_ms:.4f} ms "
+            f"({ceiling['GB_per_s']} GB/s); cloud kernel at {ceiling['cloud_kernel_frac']:.1%} "
+            f"of it")
 
         # encrypt and decrypt kernels on the same workload (outside the timed steps)
         eng.timing(True)
@@ -444,13 +472,16 @@ def main(argv=None):
         "config": {"workload": f"{args.workload}: {c.num_tags} tag SNPs, {c.num_targets} target "
                                f"SNPs, {ns} samples, {c.neighbours} neighbours, plain cloud step",
                    "num_in_cts": c.num_in_cts, "num_out_cts": c.num_outputs,
-                   "variant": variant_name, "algorithmic_bytes": alg_bytes,
+                   "variant": variant_name,
+                   "tile": [info["tile_rows"], info["tile_kblocks"], info["col_tiles"]],
+                   "algorithmic_bytes": alg_bytes,
                    "target_positions_per_s": round(c.num_targets / step_s, 1),
                    "cloud_kernel_ms": round(avg_kernel_s * 1e3, 4),
                    "variance_kernel_ms": round(var_ms / kern_n, 4),
                    "encrypt_kernel_ms": round(enc_ms / enc_n, 4),
                    "decrypt_kernel_ms": round(dec_ms / dec_n, 4)},
         "roofline": roofline,
+        "hbm_ceiling": ceiling,
         "cpu_baseline": cpu,
     }
     if forms is not None:

@@ -24,7 +24,7 @@ IHE_CONSTANT_BIDX = 0xFFFFFFFF
 IHE_NOISE_TABLE_LEN = 1280
 
 VARIANT_AUTO, VARIANT_VALU, VARIANT_WIDE, VARIANT_MFMA = 0, 1, 2, 3
-TIMER_CLOUD, TIMER_CLOUD_VAR, TIMER_ENCRYPT, TIMER_DECRYPT, TIMER_BATCH = range(5)
+TIMER_CLOUD, TIMER_CLOUD_VAR, TIMER_ENCRYPT, TIMER_DECRYPT, TIMER_BATCH, TIMER_STREAM = range(6)
 
 # Every entry point the header declares, with (restype, argtypes).
 _vp, _u32, _i32, _u64, _sz, _int = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32,
@@ -72,7 +72,8 @@ class ServerStats(ctypes.Structure):
 class BandInfo(ctypes.Structure):
     _fields_ = [("num_outputs", _u32), ("num_groups", _u32), ("num_tiles", _u32),
                 ("num_cols", _u32), ("nnz", _u32), ("variant", _u32), ("max_window", _u32),
-                ("device_bytes", _u64)]
+                ("device_bytes", _u64), ("tile_rows", _u32), ("tile_kblocks", _u32),
+                ("col_tiles", _u32)]
 
 
 SIGNATURES = {
@@ -94,6 +95,7 @@ SIGNATURES = {
     "ihe_memset": (_int, [_vp, _vp, _int, _sz]),
     "ihe_memcpy_d2d": (_int, [_vp, _vp, _vp, _sz]),
     "ihe_checksum": (_int, [_vp, _vp, _sz, ctypes.POINTER(_u64)]),
+    "ihe_stream_probe": (_int, [_vp, _vp, _sz, _vp, _sz]),
     "ihe_timing_enable": (_int, [_vp, _int]),
     "ihe_timing_reset": (_int, [_vp]),
     "ihe_timing_read": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double),

@@ -565,7 +565,9 @@ struct SegmentBuilder {
     const uint32_t nil = std::min(in_lo, ilo), nih = std::max(in_hi, ihi);
     const uint32_t nol = std::min(out_lo, olo), noh = std::max(out_hi, ohi);
     const bool in_ok = nil == UINT32_MAX || nih - nil < kSegCts;
-    if (unit > first && (!in_ok || noh - nol >= kSegCts)) {
+    // outputs stay kSegPadRows below kSegCts: a tile's padding rows store at
+    // the drop offset plus their row offset, which must stay below 2^32
+    if (unit > first && (!in_ok || noh - nol >= kSegCts - kSegPadRows)) {
       close(unit);
       in_lo = ilo, in_hi = ihi, out_lo = olo, out_hi = ohi;
     } else {
@@ -934,33 +936,47 @@ uint32_t count_tiles(uint32_t num_out, const MfmaShape &shape, const HostBand &H
   return tiles;
 }
 
+// Column tiles per wave (per workgroup for the four-row-block kernel):
+// IHE_CLOUD_NTW if it names one the kernel family has, else its default.
+int pick_ntw(int rb) {
+  const int v = (int)env_u32("IHE_CLOUD_NTW", 0, 4, 32);
+  if (rb == kMfmaWideRb) return v == 8 || v == 32 ? v : 16;
+  return v == 8 || v == 16 ? v : 4;
+}
+
 // MFMA tile shape: IHE_CLOUD_SHAPE="rb,nkb" if given, else the candidate with
-// the least estimated step work, tiles * (nkb + 1.5 * rb): a column-tile step
-// issues 16 loads per K-block and 16 stores per row block. The kernel
-// autotuner (ihe_band_tune) refines this choice by measurement.
+// the least estimated time. Per-wave kernels (rb 1, 2): tiles * (nkb + 1.5 rb),
+// their per-tile and column-tile loads (16 per K-block) and stores (16 per row
+// block). Four-row-block kernel: tiles * (3.45 nkb + 3.7), fitted on one
+// MI355X to the full 80,882-target layout (n = 5: 0.49 ms for 64-row tiles
+// against 0.59-0.62 ms for 128-row ones; n = 50: 0.92 ms for 128 rows x 6
+// K-blocks against 1.01 ms for 32 x 5). The kernel autotuner
+// (ihe_band_create_tuned) refines this choice by measurement.
 bool pick_mfma_shape(uint32_t num_out, HostBand &H, MfmaShape &shape) {
-  shape.ntw = (int)env_u32("IHE_CLOUD_NTW", 4, 4, 16);
-  if (shape.ntw != 8 && shape.ntw != 16) shape.ntw = 4;
   if (const char *e = std::getenv("IHE_CLOUD_SHAPE")) {
     int rb = 0, nkb = 0;
-    if (std::sscanf(e, "%d,%d", &rb, &nkb) == 2 && (rb == 1 || (rb == 2 && nkb == 1)) && nkb >= 1 &&
+    if (std::sscanf(e, "%d,%d", &rb, &nkb) == 2 &&
+        (rb == 1 || (rb == 2 && nkb == 1) || rb == kMfmaWideRb) && nkb >= 1 &&
         nkb <= (int)kMfmaMaxNkb) {
-      shape.rb = rb, shape.nkb = nkb;
+      shape = {rb, nkb, pick_ntw(rb)};
       return count_tiles(num_out, shape, H) > 0;
     }
   }
   const int kmin = std::max<int>(1, (int)((H.max_window + 31) / 32));
   std::vector<MfmaShape> cands;
-  if (kmin == 1) cands.push_back({2, 1, shape.ntw}), cands.push_back({1, 1, shape.ntw});
+  if (kmin == 1) cands.push_back({2, 1, pick_ntw(2)}), cands.push_back({1, 1, pick_ntw(1)});
   for (int k = std::max(2, kmin); k <= std::min<int>(kmin + 2, (int)kMfmaMaxNkb); ++k)
-    cands.push_back({1, k, shape.ntw});
+    cands.push_back({1, k, pick_ntw(1)});
+  for (int k = kmin; k <= std::min<int>(kmin + 3, (int)kMfmaMaxNkb); ++k)
+    cands.push_back({kMfmaWideRb, k, pick_ntw(kMfmaWideRb)});
   double best = 0;
   bool found = false;
   for (const MfmaShape &c : cands) {
     const uint32_t t = count_tiles(num_out, c, H);
     if (t == 0) continue;
     H.candidates.push_back(c);
-    const double cost = (double)t * (c.nkb + 1.5 * c.rb);
+    const double cost = c.rb == kMfmaWideRb ? (double)t * (3.45 * c.nkb + 3.7)
+                                            : (double)t * (c.nkb + 1.5 * c.rb);
     if (!found || cost < best) best = cost, shape = c, found = true;
   }
   return found || num_out == 0;
@@ -1074,6 +1090,10 @@ void fill_info(const HostBand &H, const ihe_model_csr *model, ihe_band_info *inf
                                                                  : IHE_VARIANT_VALU;
   info->max_window = H.max_window;
   info->device_bytes = 0;
+  const bool mf = H.mode == kModeMfma;
+  info->tile_rows = mf ? kMfmaRows * (uint32_t)H.shape.rb : 0;
+  info->tile_kblocks = mf ? (uint32_t)H.shape.nkb : 0;
+  info->col_tiles = mf ? (uint32_t)H.shape.ntw : 0;
 }
 
 template <typename T>
@@ -1217,9 +1237,11 @@ int ihe_band_create_tuned(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_i
     if ((rc = plan_band(layout, num_in_cts, model, IHE_VARIANT_MFMA, H, &cand))) break;
     ihe_band *b = nullptr;
     if ((rc = upload_band(ctx, layout, num_in_cts, model, H, &b))) break;
-    int best_ntw = 4;
+    int best_ntw = 0;
     float band_ms = 0;
-    for (int ntw : {4, 8, 16}) {
+    const std::array<int, 3> ntws =
+        cand.rb == kMfmaWideRb ? std::array<int, 3>{8, 16, 32} : std::array<int, 3>{4, 8, 16};
+    for (int ntw : ntws) {
       b->shape.ntw = ntw;
       launch_mfma_band(ctx, b, *layout, num_in_cts, d_in, d_out);  // warm-up
       (void)hipEventRecord(e0, ctx->stream);
@@ -1234,7 +1256,7 @@ int ihe_band_create_tuned(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_i
       ms /= (float)reps;
       if (n < cap) results[n] = {32u * (uint32_t)cand.rb, (uint32_t)cand.nkb, (uint32_t)ntw, ms};
       ++n;
-      if (ntw == 4 || ms < band_ms) band_ms = ms, best_ntw = ntw;
+      if (best_ntw == 0 || ms < band_ms) band_ms = ms, best_ntw = ntw;
     }
     b->shape.ntw = best_ntw;
     if (rc == IHE_OK && (!best || band_ms < best_ms)) {
@@ -1274,6 +1296,10 @@ int ihe_band_get_info(const ihe_band *band, ihe_band_info *info) {
   info->variant = (uint32_t)band->variant;
   info->max_window = band->max_window;
   info->device_bytes = band->dbytes;
+  const bool mf = band->mode == kModeMfma;
+  info->tile_rows = mf ? kMfmaRows * (uint32_t)band->shape.rb : 0;
+  info->tile_kblocks = mf ? (uint32_t)band->shape.nkb : 0;
+  info->col_tiles = mf ? (uint32_t)band->shape.ntw : 0;
   return IHE_OK;
 }
 

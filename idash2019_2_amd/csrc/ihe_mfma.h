@@ -10,6 +10,7 @@ namespace ihe {
 constexpr uint32_t kMfmaRows = 32;      // outputs per MFMA row block
 constexpr uint32_t kMfmaMaxNkb = 8;     // K-blocks (of 32 columns) per tile
 constexpr uint32_t kSegCts = (1u << 18) - 1;  // ciphertexts per segment: byte offsets < 2^31
+constexpr uint32_t kSegPadRows = 128;          // rows per tile at most
 
 typedef int int4v __attribute__((ext_vector_type(4)));
 
@@ -43,11 +44,22 @@ struct alignas(16) MfmaEntry {
 static_assert(sizeof(MfmaEntry) == 128, "entry is two 64-byte scalar loads");
 
 // Kernel family of a tiling.
+//   rb 1 or 2: each wave owns column tiles of the tile's rows (rb 2 only with
+//              nkb 1); ntw = column tiles per wave (4, 8 or 16).
+//   rb 4:      the workgroup's four waves own one row block each and share the
+//              column tiles; ntw = column tiles per workgroup (8, 16 or 32).
+constexpr int kMfmaWideRb = 4;
 struct MfmaShape {
-  int rb = 2;   // row blocks per tile (1 or 2; 2 only with nkb == 1)
+  int rb = 2;   // row blocks per tile
   int nkb = 1;  // K-blocks per tile
-  int ntw = 4;  // 32-word column tiles per wave (4, 8 or 16)
+  int ntw = 4;  // 32-word column tiles per wave (per workgroup for rb 4)
 };
+
+// Workgroups per tile of a shape (a ciphertext has kCtWords / 32 column tiles).
+inline uint32_t mfma_wg_per_tile(const MfmaShape &s) {
+  const uint32_t waves_per_tile = s.rb == kMfmaWideRb ? 1 : 4;
+  return kCtWords / (waves_per_tile * (uint32_t)s.ntw * 32);
+}
 
 // Enqueues the MFMA cloud kernel. Single cohort: entries == nullptr and
 // `single` describes it, with num_items = its tiles. Batch: entries[e] for

@@ -93,39 +93,53 @@ __device__ __forceinline__ uint32_t col_word(uint32_t c, const MfmaEntry &E) {
   }
 }
 
-// Loads word `word` (0..2047: a then b) of the lane's 16 columns; a rotated
-// column reads P[i + m] for i + m < N and -P[i + m - N] past the end.
+// Loads word `word` (0..2047: a then b) of column word cw; a rotated column
+// reads P[i + m] for i + m < N and -P[i + m - N] past the end.
+template <bool ROT>
+__device__ __forceinline__ uint32_t load_col(__amdgpu_buffer_rsrc_t in_r, uint32_t cw,
+                                             uint32_t word) {
+  if constexpr (!ROT) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(in_r, cw + word * 4u, 0, 0);
+  } else {
+    const uint32_t t = (word & (kN - 1)) + ((cw >> 2) & (kN - 1));
+    const uint32_t v = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
+        in_r, (cw & ~0x1FFFu) + ((t & (kN - 1)) << 2) + (word & kN) * 4u, 0, 0);
+    return t >= kN ? 0u - v : v;
+  }
+}
+
+// Word `word` of the lane's 16 columns.
 template <bool ROT>
 __device__ __forceinline__ void load_cols(__amdgpu_buffer_rsrc_t in_r, const uint32_t (&cw)[16],
                                           uint32_t word, uint32_t (&x)[16]) {
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    if constexpr (!ROT) {
-      x[j] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(in_r, cw[j] + word * 4u, 0, 0);
-    } else {
-      const uint32_t t = (word & (kN - 1)) + ((cw[j] >> 2) & (kN - 1));
-      const uint32_t v = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(
-          in_r, (cw[j] & ~0x1FFFu) + ((t & (kN - 1)) << 2) + (word & kN) * 4u, 0, 0);
-      x[j] = t >= kN ? 0u - v : v;
-    }
-  }
+  for (int j = 0; j < 16; ++j) x[j] = load_col<ROT>(in_r, cw[j], word);
+}
+
+// 4 words -> 4 dwords of balanced signed bytes: dword b holds digit b of
+// words 0..3, in word order.
+__device__ __forceinline__ void byte_quad(const uint32_t *x, uint32_t (&d)[4]) {
+  uint32_t y[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) y[i] = (x[i] + 0x80808080u) ^ 0x80808080u;
+  const uint32_t t0 = __builtin_amdgcn_perm(y[1], y[0], 0x05010400u);
+  const uint32_t t1 = __builtin_amdgcn_perm(y[1], y[0], 0x07030602u);
+  const uint32_t t2 = __builtin_amdgcn_perm(y[3], y[2], 0x05010400u);
+  const uint32_t t3 = __builtin_amdgcn_perm(y[3], y[2], 0x07030602u);
+  d[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+  d[1] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+  d[2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+  d[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
 }
 
 // 16 words -> 4 planes of balanced signed bytes (plane b = digit b of each word).
 __device__ __forceinline__ void byte_planes(const uint32_t (&x)[16], int4v (&p)[4]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    uint32_t y[4];
+    uint32_t d[4];
+    byte_quad(x + 4 * q, d);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) y[i] = (x[4 * q + i] + 0x80808080u) ^ 0x80808080u;
-    const uint32_t t0 = __builtin_amdgcn_perm(y[1], y[0], 0x05010400u);
-    const uint32_t t1 = __builtin_amdgcn_perm(y[1], y[0], 0x07030602u);
-    const uint32_t t2 = __builtin_amdgcn_perm(y[3], y[2], 0x05010400u);
-    const uint32_t t3 = __builtin_amdgcn_perm(y[3], y[2], 0x07030602u);
-    p[0][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x05040100u);
-    p[1][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x07060302u);
-    p[2][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x05040100u);
-    p[3][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x07060302u);
+    for (int b = 0; b < 4; ++b) p[b][q] = (int)d[b];
   }
 }
 
@@ -223,7 +237,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfma1(MfmaArgs A) {
     a[rb][0] = E.adata[(size_t)(T.a_off + 2 * rb) * 64 + lane];
     a[rb][1] = E.adata[(size_t)(T.a_off + 2 * rb + 1) * 64 + lane];
   }
+  // a wave walks NTW consecutive column tiles (interleaving them over the
+  // tile's waves measured no faster)
   const uint32_t nt0 = (part * kWaves + wave) * NTW;
+  auto col = [&](uint32_t nt) { return nt0 + nt; };
   const uint32_t row_base = T.slot0 * kCtWords * 4u;
   const bool full = T.nrows == rows;
   __syncthreads();
@@ -238,19 +255,19 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfma1(MfmaArgs A) {
     for (int rb = 0; rb < RB; ++rb) {
       Acc c = {};
       mfma_kblock(a[rb][0], a[rb][1], pl, c);
-      finish_rows(c, E, out_r, s_bias, (nt0 + nt) * 32, r, h, 32 * rb, T.nrows, full, rbase);
+      finish_rows(c, E, out_r, s_bias, col(nt) * 32, r, h, 32 * rb, T.nrows, full, rbase);
     }
   };
 
   uint32_t x[16], xn[16];
-  load_cols<ROT>(in_r, cw, nt0 * 32 + r, x);
-  load_cols<ROT>(in_r, cw, (nt0 + (NTW > 1 ? 1 : 0)) * 32 + r, xn);
+  load_cols<ROT>(in_r, cw, col(0) * 32 + r, x);
+  load_cols<ROT>(in_r, cw, col(NTW > 1 ? 1 : 0) * 32 + r, xn);
   step(x, 0);
   for (uint32_t nt = 1; nt < (uint32_t)NTW; ++nt) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) x[j] = xn[j];
     const uint32_t ntn = nt + 1 < (uint32_t)NTW ? nt + 1 : nt;
-    load_cols<ROT>(in_r, cw, (nt0 + ntn) * 32 + r, xn);
+    load_cols<ROT>(in_r, cw, col(ntn) * 32 + r, xn);
     step(x, nt);
   }
 }
@@ -288,6 +305,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
     s_a[i] = i < T.nkb * 2 * 64 ? E.adata[(size_t)T.a_off * 64 + i] : int4v{0, 0, 0, 0};
   if (threadIdx.x < kMfmaRows) s_bias[threadIdx.x] = E.rowc[T.row_off + threadIdx.x];
   const uint32_t nt0 = (part * kWaves + wave) * NTW;
+  auto col = [&](uint32_t nt) { return nt0 + nt; };
   const uint32_t row_base = T.slot0 * kCtWords * 4u;
   const bool full = T.nrows == kMfmaRows;
   __syncthreads();
@@ -315,10 +333,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
         uint32_t cw[16];
         cols_of(kb + 1 < NKB ? kb + 1 : 0, cw);
         if (kb + 1 < NKB) {
-          load_cols<ROT>(in_r, cw, (nt0 + nt) * 32 + r, xb);
+          load_cols<ROT>(in_r, cw, col(nt) * 32 + r, xb);
         } else {
           const uint32_t ntn = nt + 1 < (uint32_t)NTW ? nt + 1 : nt;
-          load_cols<ROT>(in_r, cw, (nt0 + ntn) * 32 + r, xnext);
+          load_cols<ROT>(in_r, cw, col(ntn) * 32 + r, xnext);
         }
       }
       __builtin_amdgcn_sched_barrier(0);
@@ -333,19 +351,124 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
     }
     uint32_t rbase = row_base;
     asm volatile("" : "+s"(rbase));
-    finish_rows(c, E, out_r, s_bias, (nt0 + nt) * 32, r, h, 0, T.nrows, full, rbase);
+    finish_rows(c, E, out_r, s_bias, col(nt) * 32, r, h, 0, T.nrows, full, rbase);
 #pragma unroll
     for (int j = 0; j < 16; ++j) xa[j] = xnext[j];
   };
   {
     uint32_t cw[16];
     cols_of(0, cw);
-    load_cols<ROT>(in_r, cw, nt0 * 32 + r, xa);
+    load_cols<ROT>(in_r, cw, col(0) * 32 + r, xa);
   }
   // the first column tile is peeled so that the loop is entered, like it is
   // re-entered, with 16 loads followed by 16 stores outstanding
   column_tile(0);
   for (uint32_t nt = 1; nt < (uint32_t)NTW; ++nt) column_tile(nt);
+}
+
+// ---------------------------------------------------------------------------
+// Four row blocks per tile (128 rows), one per wave, over NKB K-blocks (wide
+// windows). The workgroup recodes each 32-word column tile of its columns into
+// byte planes once, in LDS, and the four waves take their B operands from
+// there, so an input word is loaded once per 128 output rows. Loading is
+// spread evenly: for every K-block, lane t of wave w fetches 4 columns of one
+// word, those that make up dword t % 4 of the B operand of lane
+// 16 w + t / 4, and writes that dword's four planes, so both the LDS writes
+// and the B operand reads are lane-linear. LDS holds two column tiles: the
+// waves multiply one while the next one's loads are in flight. Per column tile
+// a lane issues 16 stores then 4*NKB loads, in a fixed order, and the
+// workgroup meets at one barrier.
+template <bool ROT, int NTW, int NKB>
+__global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmaw(MfmaArgs A) {
+  constexpr uint32_t wg_per_tile = kCtWords / (NTW * 32);
+  constexpr uint32_t rows = kMfmaRows * kWaves;
+  __shared__ uint32_t s_b[2][NKB * 4 * 64 * 4];  // [buffer][((kb * 4 + plane) * 64 + lane) * 4 + q]
+  __shared__ uint32_t s_bias[rows];
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  uint32_t tile;
+  const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
+  const uint32_t nt0 = (blk % wg_per_tile) * NTW;
+  auto col = [&](uint32_t nt) { return nt0 + nt; };
+  // the wave index is uniform: row offsets stay scalar
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  const uint32_t h = lane / 32, r = lane % 32;
+  const MTileDev T = tile_of(E, tile);
+  const __amdgpu_buffer_rsrc_t in_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)E.in, (short)0, (int)E.in_bytes, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t out_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)E.out, (short)0, (int)E.out_bytes, kRsrcWord3);
+  if (threadIdx.x < rows) s_bias[threadIdx.x] = E.rowc[T.row_off + threadIdx.x];
+  // this wave's A operands; K-blocks past the tile's own are zero
+  int4v a[NKB][2];
+#pragma unroll
+  for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+    for (int hp = 0; hp < 2; ++hp)
+      a[kb][hp] = (uint32_t)kb < T.nkb
+                      ? E.adata[(size_t)(T.a_off + (wave * T.nkb + kb) * 2 + hp) * 64 + lane]
+                      : int4v{0, 0, 0, 0};
+  // the columns this lane loads: 4 per K-block, for B lane L = 16 w + t / 4,
+  // dword q = t % 4 (columns 16 (L / 32) + 4 q + i of the K-block, word L % 32);
+  // padding K-blocks repeat the tile's last K-block
+  const uint32_t L = 16 * wave + lane / 4, q = lane % 4;
+  const uint32_t lw = L % 32;
+  uint32_t cw[NKB][4];
+  {
+    const IHE_CONSTANT uint32_t *kc = cptr(E.kcols) + T.kcol_off + 16 * (L / 32) + 4 * q;
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) {
+      const uint32_t kbs = (uint32_t)kb < T.nkb ? (uint32_t)kb : T.nkb - 1;
+      const u32x4v c4 = *(const IHE_CONSTANT u32x4v *)(kc + 32 * kbs);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cw[kb][i] = col_word<ROT>(c4[i], E);
+    }
+  }
+  const uint32_t row_base = T.slot0 * kCtWords * 4u;
+  const bool full = T.nrows == rows;
+
+  uint32_t x[NKB][4];
+  auto load_tile = [&](uint32_t nt) {
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[kb][i] = load_col<ROT>(in_r, cw[kb][i], col(nt) * 32 + lw);
+  };
+  auto stage_tile = [&](uint32_t buf) {
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) {
+      uint32_t d[4];
+      byte_quad(x[kb], d);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) s_b[buf][((kb * 4 + b) * 64 + L) * 4 + q] = d[b];
+    }
+  };
+  auto multiply_tile = [&](uint32_t nt, uint32_t buf) {
+    Acc c = {};
+    const int4v *sb = (const int4v *)s_b[buf];
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) {
+      int4v pl[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) pl[b] = sb[(kb * 4 + b) * 64 + lane];
+      mfma_kblock(a[kb][0], a[kb][1], pl, c);
+    }
+    uint32_t rbase = row_base;
+    asm volatile("" : "+s"(rbase));
+    finish_rows(c, E, out_r, s_bias, col(nt) * 32, r, h, 32 * wave, T.nrows, full, rbase);
+  };
+
+  load_tile(0);
+  stage_tile(0);
+  load_tile(NTW > 1 ? 1 : 0);
+  __syncthreads();
+  for (uint32_t nt = 0; nt < (uint32_t)NTW; ++nt) {
+    multiply_tile(nt, nt & 1);
+    // the loads of column tile nt + 1 (the last tile reloads itself) were
+    // issued before this tile's stores
+    stage_tile((nt + 1) & 1);
+    load_tile(nt + 2 < (uint32_t)NTW ? nt + 2 : NTW - 1);
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -404,9 +527,28 @@ void launch_kk(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
   }
 }
 
+template <bool ROT, int NTW>
+void launch_kw(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
+  const dim3 block(64 * kWaves);
+  switch (nkb) {
+    case 1: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 1>), grid, block, 0, s, A); break;
+    case 2: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 2>), grid, block, 0, s, A); break;
+    case 3: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 3>), grid, block, 0, s, A); break;
+    case 4: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 4>), grid, block, 0, s, A); break;
+    case 5: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 5>), grid, block, 0, s, A); break;
+    case 6: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 6>), grid, block, 0, s, A); break;
+    case 7: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 7>), grid, block, 0, s, A); break;
+    default: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 8>), grid, block, 0, s, A); break;
+  }
+}
+
 template <bool ROT>
 void launch_rot(hipStream_t s, const MfmaShape &sh, dim3 grid, const MfmaArgs &A) {
-  if (sh.nkb == 1) {
+  if (sh.rb == kMfmaWideRb) {
+    if (sh.ntw == 32) launch_kw<ROT, 32>(s, sh.nkb, grid, A);
+    else if (sh.ntw == 16) launch_kw<ROT, 16>(s, sh.nkb, grid, A);
+    else launch_kw<ROT, 8>(s, sh.nkb, grid, A);
+  } else if (sh.nkb == 1) {
     if (sh.ntw == 16) launch_k1<ROT, 16>(s, sh.rb, grid, A);
     else if (sh.ntw == 8) launch_k1<ROT, 8>(s, sh.rb, grid, A);
     else launch_k1<ROT, 4>(s, sh.rb, grid, A);
@@ -423,7 +565,7 @@ void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const Mfm
                  const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
                  uint32_t num_items) {
   if (num_items == 0) return;
-  const uint32_t wg_per_tile = kCtWords / (kWaves * (uint32_t)shape.ntw * 32);
+  const uint32_t wg_per_tile = mfma_wg_per_tile(shape);
   const MfmaArgs A = {single, entries, first, num_entries};
   const dim3 grid(num_items * wg_per_tile);
   if (rot) launch_rot<true>(stream, shape, grid, A);

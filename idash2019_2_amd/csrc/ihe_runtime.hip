@@ -248,8 +248,9 @@ __global__ __launch_bounds__(256) void k_copy16(const u32x4 *__restrict__ src, u
     __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
-// Checksum partials: word i of the buffer contributes splitmix64((i << 32) ^ w_i),
-// summed mod 2^64 per block; the host adds the block partials.
+// Checksum partials: word i of the buffer contributes mix((i << 32) ^ w_i), mix
+// being the output mix of splitmix64, summed mod 2^64 per block; the host adds
+// the block partials.
 constexpr uint32_t kSumBlocks = 1024;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
@@ -276,9 +277,50 @@ __global__ __launch_bounds__(256) void k_checksum(const uint32_t *__restrict__ p
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// One wave per 64 KiB of output: 64 stores of 1 KiB, with the wave's share
+// of the input read in 1 KiB loads spread evenly between them.
+__global__ __launch_bounds__(256) void k_stream_probe(const int *in, uint32_t in_bytes, int *out,
+                                                      uint32_t out_bytes, uint32_t num_waves) {
+  typedef int int4v __attribute__((ext_vector_type(4)));
+  const uint32_t wave = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (wave >= num_waves) return;
+  const uint32_t in_kib = in_bytes / 1024;
+  const uint32_t r0 = (uint32_t)((uint64_t)in_kib * wave / num_waves);
+  const uint32_t r1 = (uint32_t)((uint64_t)in_kib * (wave + 1) / num_waves);
+  const __amdgpu_buffer_rsrc_t ri =
+      __builtin_amdgcn_make_buffer_rsrc((void *)in, (short)0, (int)in_bytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t ro =
+      __builtin_amdgcn_make_buffer_rsrc((void *)out, (short)0, (int)out_bytes, 0x00020000);
+  int4v acc = {(int)lane, 0, 0, 0};
+  uint32_t r = r0;
+  for (uint32_t j = 0; j < 64; ++j) {
+    const uint32_t until = r0 + (uint32_t)((uint64_t)(r1 - r0) * (j + 1) / 64);
+    for (; r < until; ++r) acc += __builtin_amdgcn_raw_buffer_load_b128(ri, r * 1024u + lane * 16u, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(acc, ro, (wave * 64u + j) * 1024u + lane * 16u, 0, 0);
+  }
+}
+
 }  // namespace ihe
 
 extern "C" {
+
+int ihe_stream_probe(ihe_ctx *ctx, const void *d_in, size_t in_bytes, void *d_out,
+                     size_t out_bytes) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if ((in_bytes && !d_in) || !d_out || out_bytes < 65536 || in_bytes % 1024 || out_bytes % 1024 ||
+      in_bytes >= (1ull << 31) || out_bytes >= (1ull << 31) ||
+      ((uintptr_t)d_in | (uintptr_t)d_out) % 16)
+    return fail(IHE_ERR_ARG, "stream probe: sizes must be multiples of 1 KiB below 2^31 "
+                             "(at least 64 KiB written), buffers 16-byte aligned");
+  IHE_HIP(hipSetDevice(ctx->device));
+  const uint32_t waves = (uint32_t)(out_bytes / 65536);
+  ScopedTimer t(ctx, IHE_TIMER_STREAM);
+  hipLaunchKernelGGL(k_stream_probe, dim3((waves + 3) / 4), dim3(256), 0, ctx->stream,
+                     (const int *)d_in, (uint32_t)in_bytes, (int *)d_out, (uint32_t)out_bytes,
+                     waves);
+  IHE_HIP(hipGetLastError());
+  return IHE_OK;
+}
 
 int ihe_checksum(ihe_ctx *ctx, const void *d_ptr, size_t bytes, uint64_t *out) {
   if (int rc = check_ctx(ctx)) return rc;

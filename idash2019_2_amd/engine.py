@@ -290,6 +290,14 @@ class Engine:
                                  ctypes.byref(out)))
         return out.value
 
+    def stream_probe(self, d_in: DeviceArray, in_bytes: int, d_out: DeviceArray,
+                     out_bytes: int) -> None:
+        """ihe_stream_probe: reads in_bytes of d_in and writes out_bytes of d_out
+        (HBM bandwidth of a read:write mix); timed under TIMER_STREAM."""
+        if in_bytes > d_in.nbytes or out_bytes > d_out.nbytes:
+            raise ValueError("probe sizes exceed the buffers")
+        check(lib().ihe_stream_probe(self.ctx, d_in.ptr, in_bytes, d_out.ptr, out_bytes))
+
     def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int | None = None) -> None:
         n = min(dst.nbytes, src.nbytes) if nbytes is None else nbytes
         check(lib().ihe_memcpy_d2d(self.ctx, dst.ptr, src.ptr, n))

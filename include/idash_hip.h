@@ -66,7 +66,7 @@ typedef struct {
 
 /* Cloud kernel variants. */
 enum {
-  IHE_VARIANT_AUTO = 0,  /* pick per band (dot2 VALU unless the band is wide and MFMA wins) */
+  IHE_VARIANT_AUTO = 0,  /* MFMA when the weights fit its byte split, else VALU or WIDE */
   IHE_VARIANT_VALU = 1,  /* int16-pair dot2 VALU kernel (weights within int16) */
   IHE_VARIANT_WIDE = 2,  /* int32 weights, 32-bit multiply VALU kernel */
   IHE_VARIANT_MFMA = 3   /* byte-split int8 MFMA kernel (weights within int16) */
@@ -77,6 +77,10 @@ typedef struct {
   uint32_t variant;        /* the variant the band was compiled for */
   uint32_t max_window;     /* widest column list of a group */
   uint64_t device_bytes;   /* device memory the band holds */
+  /* MFMA bands (0 otherwise): output rows per tile (32, 64 or 128), K-blocks of
+     32 columns per tile, and 32-word column tiles per wave (rows <= 64) or per
+     workgroup (128 rows) */
+  uint32_t tile_rows, tile_kblocks, col_tiles;
 } ihe_band_info;
 
 /* ---------------- errors, devices, context ---------------- */
@@ -104,9 +108,20 @@ int ihe_memset(ihe_ctx *ctx, void *dst, int value, size_t bytes);
 int ihe_memcpy_d2d(ihe_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* Order-sensitive 64-bit checksum of a device buffer (bytes % 4 == 0):
- * sum over words i of splitmix64((i << 32) ^ w_i) mod 2^64. Compares large
+ * sum over words i of mix((i << 32) ^ w_i) mod 2^64, mix being splitmix64's
+ * output mix. Compares large
  * outputs of two runs without copying them to the host. Synchronizes. */
 int ihe_checksum(ihe_ctx *ctx, const void *d_ptr, size_t bytes, uint64_t *out);
+
+/* HBM streaming probe: reads in_bytes from d_in and writes out_bytes to d_out
+ * (both multiples of 1024 and below 2^31, d_in and d_out 16-byte aligned),
+ * contiguous 16-byte accesses, reads and writes interleaved in every wave in
+ * the ratio of the two sizes. Timed under IHE_TIMER_STREAM, it measures the
+ * bandwidth the device sustains for a read:write mix, such as the cloud
+ * step's (input ciphertexts read once, outputs written once). The written
+ * bytes are unspecified. Asynchronous. */
+int ihe_stream_probe(ihe_ctx *ctx, const void *d_in, size_t in_bytes, void *d_out,
+                     size_t out_bytes);
 
 /* ---------------- timing of the library's own kernels ---------------- */
 enum {
@@ -115,7 +130,8 @@ enum {
   IHE_TIMER_ENCRYPT = 2,
   IHE_TIMER_DECRYPT = 3,
   IHE_TIMER_BATCH = 4,
-  IHE_TIMER_COUNT = 5
+  IHE_TIMER_STREAM = 5,    /* ihe_stream_probe */
+  IHE_TIMER_COUNT = 6
 };
 /* When enabled, each launch is bracketed by HIP events on the launch stream. */
 int ihe_timing_enable(ihe_ctx *ctx, int on);

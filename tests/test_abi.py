@@ -89,25 +89,48 @@ def test_band_plan_groups_shared_windows(monkeypatch):
     assert used <= mfma["num_cols"] <= 32 * mfma["num_tiles"]
     # default: single-K-block tiles of up to 64 rows, cut short where the
     # union passes 32 columns
+    assert (mfma["tile_rows"], mfma["tile_kblocks"], mfma["col_tiles"]) == (32, 4, 4)
+    # four row blocks per workgroup: tiles of 128 rows; 256 columns hold the
+    # union of 128 consecutive outputs here
+    monkeypatch.setenv("IHE_CLOUD_SHAPE", "4,8")
+    monkeypatch.setenv("IHE_CLOUD_NTW", "32")
+    w = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
+    assert (w["variant"], w["tile_rows"], w["tile_kblocks"], w["col_tiles"]) == \
+        (_abi.VARIANT_MFMA, 128, 8, 32)
+    assert w["num_tiles"] == -(-c.num_outputs // 128)
+    monkeypatch.setenv("IHE_CLOUD_NTW", "4")            # not a column-tile count of that kernel
+    assert band_plan(c.num_samples, c.num_in_cts, *c.model_csr())["col_tiles"] == 16
+    monkeypatch.delenv("IHE_CLOUD_NTW")
+    # one K-block: tiles are cut where the union passes 32 columns
+    monkeypatch.setenv("IHE_CLOUD_SHAPE", "4,1")
+    w1 = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
+    assert w1["tile_rows"] == 128 and w1["num_tiles"] > w["num_tiles"]
     monkeypatch.delenv("IHE_CLOUD_KERNEL")
     monkeypatch.delenv("IHE_CLOUD_SHAPE")
     k1 = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
     assert k1["variant"] == _abi.VARIANT_MFMA
-    assert -(-c.num_outputs // 64) <= k1["num_tiles"] <= mfma["num_tiles"]
-    assert used <= k1["num_cols"] <= 32 * k1["num_tiles"]
+    assert -(-c.num_outputs // 128) <= k1["num_tiles"] <= mfma["num_tiles"]
+    assert used <= k1["num_cols"] <= 32 * k1["tile_kblocks"] * k1["num_tiles"]
 
 
 def test_band_plan_wide_window_picks_multi_kblock_tiles(monkeypatch):
     c = synth.generate(200, 400, 1024, 50, seed=8)      # 150 columns per output
     info = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
     assert info["variant"] == _abi.VARIANT_MFMA and info["max_window"] == 150
-    # tiles of <= 32 rows over >= 5 K-blocks
-    assert info["num_tiles"] >= -(-c.num_outputs // 32)
-    monkeypatch.setenv("IHE_CLOUD_SHAPE", "1,4")       # 128 columns cannot hold a row
-    assert band_plan(c.num_samples, c.num_in_cts, *c.model_csr())["variant"] == _abi.VARIANT_VALU
-    with pytest.raises(IheError) as e:
-        band_plan(c.num_samples, c.num_in_cts, *c.model_csr(), variant=_abi.VARIANT_MFMA)
-    assert e.value.code == _abi.IHE_ERR_UNSUPPORTED
+    # tiles over >= 5 K-blocks
+    assert info["num_tiles"] >= -(-c.num_outputs // info["tile_rows"])
+    assert info["tile_kblocks"] >= 5
+    for shape in ("1,4", "4,4"):                       # 128 columns cannot hold a row
+        monkeypatch.setenv("IHE_CLOUD_SHAPE", shape)
+        assert band_plan(c.num_samples, c.num_in_cts, *c.model_csr())["variant"] == \
+            _abi.VARIANT_VALU
+        with pytest.raises(IheError) as e:
+            band_plan(c.num_samples, c.num_in_cts, *c.model_csr(), variant=_abi.VARIANT_MFMA)
+        assert e.value.code == _abi.IHE_ERR_UNSUPPORTED
+    monkeypatch.setenv("IHE_CLOUD_SHAPE", "4,6")
+    w = band_plan(c.num_samples, c.num_in_cts, *c.model_csr())
+    assert (w["variant"], w["tile_rows"], w["tile_kblocks"]) == (_abi.VARIANT_MFMA, 128, 6)
+    assert w["num_tiles"] >= -(-c.num_outputs // 128)
 
 
 def test_band_plan_wide_weights_select_wide_variant():

@@ -19,6 +19,10 @@ KERNELS = {
     "mfma-k2": {"IHE_CLOUD_SHAPE": "1,2", "IHE_CLOUD_NTW": "8"},
     "mfma-k3": {"IHE_CLOUD_SHAPE": "1,3"},
     "mfma-ntw16": {"IHE_CLOUD_NTW": "16"},
+    "mfma-rb2": {"IHE_CLOUD_SHAPE": "2,1"},
+    "mfma-w1": {"IHE_CLOUD_SHAPE": "4,1", "IHE_CLOUD_NTW": "8"},
+    "mfma-w2": {"IHE_CLOUD_SHAPE": "4,2"},
+    "mfma-w7": {"IHE_CLOUD_SHAPE": "4,7", "IHE_CLOUD_NTW": "32"},
     "staged": {"IHE_CLOUD_KERNEL": "staged"},
     "direct": {"IHE_CLOUD_KERNEL": "direct"},
 }
@@ -134,7 +138,7 @@ def test_wide_window_matches_oracle(engine, kernel):
     rp, ib, w = c.model_csr()
     out, ov, info = run_cloud(engine, ct, var, rp, ib, w, 1024)
     assert info["max_window"] == 150
-    if kernel in ("mfma", "mfma-ntw16"):
+    if kernel in ("mfma", "mfma-ntw16", "mfma-w7"):
         assert info["variant"] == _abi.VARIANT_MFMA
     ref, rv = oracle.cloud(ct, var, rp, ib, w, *c.layout)
     assert (out == ref).all() and (ov == rv).all()
@@ -184,14 +188,18 @@ def test_full_size_rows_match_oracle(engine, kernel):
     assert (ov == w2).all()
 
 
+@pytest.mark.parametrize("shape", ["2,1", "4,2", "1,2"])
 @pytest.mark.parametrize("ns", [1024, 128, 341, 16, 1])
-def test_one_band_serves_every_layout(engine, ns):
+def test_one_band_serves_every_layout(engine, ns, shape, monkeypatch):
     """An MFMA band compiled once (its columns are big indices) runs cohorts of
     any sample count: ciphertext c / R, rotation (c % R) * RS per launch."""
+    monkeypatch.setenv("IHE_CLOUD_SHAPE", shape)
     c = synth.generate(40, 120, 1024, 5, seed=3)
     rp, ib, w = c.model_csr()
     band = engine.band(1024, c.num_in_cts, rp, ib, w)
-    assert band.info()["variant"] == _abi.VARIANT_MFMA
+    info = band.info()
+    assert info["variant"] == _abi.VARIANT_MFMA
+    assert (info["tile_rows"], info["tile_kblocks"]) == (32 * int(shape[0]), int(shape[2]))
     R = 1024 // ns
     num_in = (3 * c.num_tags + R - 1) // R
     rng = np.random.default_rng(ns)

@@ -1,0 +1,59 @@
+"""Runtime utilities on the GPU: the device checksum (ihe_checksum) against its
+closed form computed on the host, and the HBM streaming probe
+(ihe_stream_probe): timed, bounded to the bytes it is given, argument errors."""
+import numpy as np
+import pytest
+
+from idash2019_2_amd import _abi
+from idash2019_2_amd._abi import IheError
+
+pytestmark = pytest.mark.gpu
+
+
+def host_checksum(words: np.ndarray) -> int:
+    """Sum over words i of mix((i << 32) ^ w_i) mod 2^64, mix being splitmix64's
+    output mix (shifts 30/27/31, multipliers 0xBF58476D1CE4E5B9 and
+    0x94D049BB133111EB)."""
+    x = (np.arange(words.size, dtype=np.uint64) << np.uint64(32)) ^ words.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        x ^= x >> np.uint64(30)
+        x *= np.uint64(0xBF58476D1CE4E5B9)
+        x ^= x >> np.uint64(27)
+        x *= np.uint64(0x94D049BB133111EB)
+        x ^= x >> np.uint64(31)
+        return int(x.sum(dtype=np.uint64))
+
+
+def test_checksum_matches_closed_form_and_sees_order(engine):
+    rng = np.random.default_rng(5)
+    w = rng.integers(0, 1 << 32, 1_000_003, dtype=np.uint64).astype(np.uint32)
+    d = engine.to_device(w)
+    assert engine.checksum(d) == host_checksum(w)
+    assert engine.checksum(d, 4 * 1000) == host_checksum(w[:1000])
+    swapped = w.copy()
+    swapped[[10, 11]] = swapped[[11, 10]]
+    assert w[10] != w[11]
+    assert engine.checksum(engine.to_device(swapped)) != host_checksum(w)
+    assert engine.checksum(d, 0) == 0
+    with pytest.raises(IheError) as e:
+        engine.checksum(d, 6)
+    assert e.value.code == _abi.IHE_ERR_ARG
+
+
+def test_stream_probe_is_timed_and_stays_in_bounds(engine):
+    in_b, out_b = 3 << 20, (5 << 20) + 65536
+    d_in = engine.to_device(np.arange(in_b // 4, dtype=np.uint32))
+    guard = 1 << 16
+    d_out = engine.to_device(np.full(out_b + guard, 0x5A, np.uint8))
+    engine.timing(True)
+    engine.timing_reset()
+    engine.stream_probe(d_in, in_b, d_out, out_b)
+    ms, n = engine.timing_read(_abi.TIMER_STREAM)
+    engine.timing(False)
+    assert n == 1 and ms > 0
+    out = d_out.numpy()
+    assert (out[out_b:] == 0x5A).all()              # nothing written past out_bytes
+    assert (out[:out_b] != 0x5A).any()
+    for args in ((in_b + 1, out_b), (in_b, 1000), (in_b, out_b + 512), (in_b, 1 << 31)):
+        with pytest.raises((IheError, ValueError)):
+            engine.stream_probe(d_in, args[0], d_out, args[1])
