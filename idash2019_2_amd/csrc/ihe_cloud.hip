@@ -1423,8 +1423,8 @@ int ihe_cloud_score_layout(ihe_ctx *ctx, const ihe_band *band, const ihe_layout 
   IHE_HIP(hipGetLastError());
   if (d_out_var) {
     ScopedTimer t(ctx, IHE_TIMER_CLOUD_VAR);
-    const VarEntry V = {band->d_var_ptr, band->d_var_col, band->d_var_w2, d_in_var, d_out_var,
-                        band->num_out, L.num_regions, {0, 0}};
+    const VarEntry V = make_var_entry(band->d_var_ptr, band->d_var_col, band->d_var_w2, d_in_var,
+                                      d_out_var, band->num_out, L.num_regions);
     launch_var(ctx->stream, V, nullptr, nullptr, 0, (band->num_out + 255) / 256);
     IHE_HIP(hipGetLastError());
   }

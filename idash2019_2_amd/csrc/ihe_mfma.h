@@ -82,8 +82,18 @@ struct VarEntry {
   const double *var_in;
   double *var_out;
   uint32_t num_out, R;
-  uint32_t pad[2];
+  uint64_t rmagic;          // c / R = high 64 bits of c * rmagic (R > 1)
 };
+
+// The variance entry of one cohort: rmagic = floor((2^64 - 1) / R) + 1 makes
+// the high half of c * rmagic equal floor(c / R) for every 32-bit c and
+// 1 < R <= 2^31 (the error c * (rmagic - 2^64 / R) / 2^64 stays below 2^-32,
+// under the 1 / R gap of a non-integral quotient to the next integer).
+inline VarEntry make_var_entry(const uint32_t *blk_off, const uint32_t *col, const int32_t *w2,
+                               const double *var_in, double *var_out, uint32_t num_out,
+                               uint32_t R) {
+  return {blk_off, col, w2, var_in, var_out, num_out, R, R > 1 ? UINT64_MAX / R + 1 : 0};
+}
 void launch_var(hipStream_t stream, const VarEntry &single, const VarEntry *entries,
                 const uint32_t *first, uint32_t num_entries, uint32_t num_blocks);
 

@@ -480,6 +480,13 @@ struct VarArgs {
   uint32_t num_entries;
 };
 
+__device__ __forceinline__ uint32_t div_r(uint32_t c, uint32_t R, uint64_t rmagic) {
+  return R == 1 ? c : (uint32_t)__umul64hi((uint64_t)c, rmagic);
+}
+
+// One wave per 64 outputs, lane = output. Entries are taken 8 at a time: their
+// column, weight and input-variance loads are all issued before the first
+// add, and the adds then run in entry order (the order of the model's row).
 __global__ __launch_bounds__(256) void k_cloud_var(VarArgs A) {
   VarEntry E = A.single;
   uint32_t chunk = blockIdx.x;
@@ -496,13 +503,38 @@ __global__ __launch_bounds__(256) void k_cloud_var(VarArgs A) {
   const uint32_t blk = chunk * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
   const uint32_t o = blk * 64 + lane;
   if (blk * 64 >= E.num_out) return;
+  constexpr int kU = 8;
   double v = 0.0;
   const uint32_t e1 = E.blk_off[blk + 1];
-  for (uint32_t e = E.blk_off[blk]; e < e1; ++e) {
+  uint32_t e = E.blk_off[blk];
+  auto take = [&](uint32_t c, bool &keep, double &x) {
+    const uint32_t ct = div_r(c, E.R, E.rmagic);
+    keep = c != kVarPad && c == ct * E.R;
+    x = keep ? E.var_in[ct] : 0.0;
+  };
+  for (; e + kU <= e1; e += kU) {
+    uint32_t c[kU];
+    int32_t w2[kU];
+#pragma unroll
+    for (int j = 0; j < kU; ++j) {
+      c[j] = E.col[(size_t)(e + j) * 64 + lane];
+      w2[j] = E.w2[(size_t)(e + j) * 64 + lane];
+    }
+    bool keep[kU];
+    double x[kU];
+#pragma unroll
+    for (int j = 0; j < kU; ++j) take(c[j], keep[j], x[j]);
+#pragma unroll
+    for (int j = 0; j < kU; ++j)
+      if (keep[j]) v = __dadd_rn(v, __dmul_rn((double)w2[j], x[j]));
+  }
+  for (; e < e1; ++e) {
     const uint32_t c = E.col[(size_t)e * 64 + lane];
-    const uint32_t ct = c / E.R;
-    if (c != kVarPad && c == ct * E.R)
-      v = __dadd_rn(v, __dmul_rn((double)E.w2[(size_t)e * 64 + lane], E.var_in[ct]));
+    const int32_t w2 = E.w2[(size_t)e * 64 + lane];
+    bool keep;
+    double x;
+    take(c, keep, x);
+    if (keep) v = __dadd_rn(v, __dmul_rn((double)w2, x));
   }
   if (o < E.num_out) E.var_out[o] = v;
 }
