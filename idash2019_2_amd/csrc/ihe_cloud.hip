@@ -947,11 +947,11 @@ int pick_ntw(int rb) {
 // MFMA tile shape: IHE_CLOUD_SHAPE="rb,nkb" if given, else the candidate with
 // the least estimated time. Per-wave kernels (rb 1, 2): tiles * (nkb + 1.5 rb),
 // their per-tile and column-tile loads (16 per K-block) and stores (16 per row
-// block). Four-row-block kernel: tiles * (3.45 nkb + 3.7), fitted on one
-// MI355X to the full 80,882-target layout (n = 5: 0.49 ms for 64-row tiles
-// against 0.59-0.62 ms for 128-row ones; n = 50: 0.92 ms for 128 rows x 6
-// K-blocks against 1.01 ms for 32 x 5). The kernel autotuner
-// (ihe_band_create_tuned) refines this choice by measurement.
+// block). Four-row-block kernel: tiles * (3.2 nkb + 4.5), fitted on one
+// MI355X to the full 80,882-target layout (n = 5: 0.48 ms for 64-row tiles
+// against 0.61 ms for 128 rows x 2 K-blocks; n = 50: 0.88 ms for 128 rows x 6
+// K-blocks, 0.98 ms for 128 x 8, against 1.01 ms for 32 x 5). The kernel
+// autotuner (ihe_band_create_tuned) refines this choice by measurement.
 bool pick_mfma_shape(uint32_t num_out, HostBand &H, MfmaShape &shape) {
   if (const char *e = std::getenv("IHE_CLOUD_SHAPE")) {
     int rb = 0, nkb = 0;
@@ -967,7 +967,8 @@ bool pick_mfma_shape(uint32_t num_out, HostBand &H, MfmaShape &shape) {
   if (kmin == 1) cands.push_back({2, 1, pick_ntw(2)}), cands.push_back({1, 1, pick_ntw(1)});
   for (int k = std::max(2, kmin); k <= std::min<int>(kmin + 2, (int)kMfmaMaxNkb); ++k)
     cands.push_back({1, k, pick_ntw(1)});
-  for (int k = kmin; k <= std::min<int>(kmin + 3, (int)kMfmaMaxNkb); ++k)
+  // the four-row-block kernel runs an even number of K-blocks
+  for (int k = kmin + kmin % 2; k <= std::min<int>(kmin + 3, (int)kMfmaMaxNkb); k += 2)
     cands.push_back({kMfmaWideRb, k, pick_ntw(kMfmaWideRb)});
   double best = 0;
   bool found = false;
@@ -975,7 +976,7 @@ bool pick_mfma_shape(uint32_t num_out, HostBand &H, MfmaShape &shape) {
     const uint32_t t = count_tiles(num_out, c, H);
     if (t == 0) continue;
     H.candidates.push_back(c);
-    const double cost = c.rb == kMfmaWideRb ? (double)t * (3.45 * c.nkb + 3.7)
+    const double cost = c.rb == kMfmaWideRb ? (double)t * (3.2 * c.nkb + 4.5)
                                             : (double)t * (c.nkb + 1.5 * c.rb);
     if (!found || cost < best) best = cost, shape = c, found = true;
   }

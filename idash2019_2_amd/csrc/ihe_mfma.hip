@@ -368,21 +368,27 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
 
 // ---------------------------------------------------------------------------
 // Four row blocks per tile (128 rows), one per wave, over NKB K-blocks (wide
-// windows). The workgroup recodes each 32-word column tile of its columns into
-// byte planes once, in LDS, and the four waves take their B operands from
-// there, so an input word is loaded once per 128 output rows. Loading is
-// spread evenly: for every K-block, lane t of wave w fetches 4 columns of one
-// word, those that make up dword t % 4 of the B operand of lane
-// 16 w + t / 4, and writes that dword's four planes, so both the LDS writes
-// and the B operand reads are lane-linear. LDS holds two column tiles: the
-// waves multiply one while the next one's loads are in flight. Per column tile
-// a lane issues 16 stores then 4*NKB loads, in a fixed order, and the
-// workgroup meets at one barrier.
+// windows; NKB even). The workgroup recodes each 32-word column tile of its
+// columns into byte planes once, in LDS, and the four waves take their B
+// operands from there, so an input word is loaded once per 128 output rows.
+//
+// Loading is split in half K-blocks (16 columns x 32 words): a lane loads the
+// 16 columns of one word, so each load instruction reads two columns' 128-byte
+// lines, and writes the 16-byte B operand of one lane per plane, lane-linear.
+// A column tile has 2 NKB half K-blocks: in each of NKB / 4 full rounds a wave
+// takes two (one per half-wave); when NKB / 2 is odd a last round gives each
+// wave one, which both its halves load (the second half's copy is not written).
+// LDS holds two column tiles: while the waves multiply one, the next is recoded
+// and the one after is being loaded. Per column tile a lane issues 16 loads per
+// round and 16 stores, in a fixed order, and the workgroup meets at one barrier.
 template <bool ROT, int NTW, int NKB>
 __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmaw(MfmaArgs A) {
+  static_assert(NKB % 2 == 0, "the four waves split NKB / 2 K-block pairs");
   constexpr uint32_t wg_per_tile = kCtWords / (NTW * 32);
   constexpr uint32_t rows = kMfmaRows * kWaves;
-  __shared__ uint32_t s_b[2][NKB * 4 * 64 * 4];  // [buffer][((kb * 4 + plane) * 64 + lane) * 4 + q]
+  constexpr int kFull = NKB / 4;                    // rounds of two half K-blocks per wave
+  constexpr int kRounds = kFull + (NKB % 4 ? 1 : 0);
+  __shared__ int4v s_b[2][NKB * 4 * 64];  // [buffer][(kb * 4 + plane) * 64 + B lane]
   __shared__ uint32_t s_bias[rows];
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   uint32_t tile;
@@ -407,68 +413,97 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmaw(MfmaArgs A) {
       a[kb][hp] = (uint32_t)kb < T.nkb
                       ? E.adata[(size_t)(T.a_off + (wave * T.nkb + kb) * 2 + hp) * 64 + lane]
                       : int4v{0, 0, 0, 0};
-  // the columns this lane loads: 4 per K-block, for B lane L = 16 w + t / 4,
-  // dword q = t % 4 (columns 16 (L / 32) + 4 q + i of the K-block, word L % 32);
-  // padding K-blocks repeat the tile's last K-block
-  const uint32_t L = 16 * wave + lane / 4, q = lane % 4;
-  const uint32_t lw = L % 32;
-  uint32_t cw[NKB][4];
+  // round rr: K-block and half (16 columns) this lane loads, for word r
+  auto kb_of = [&](int rr) -> uint32_t { return rr < kFull ? 4 * rr + wave : 4 * kFull + wave / 2; };
+  auto half_of = [&](int rr) -> uint32_t { return rr < kFull ? h : wave % 2; };
+  // column words; K-blocks past the tile's own repeat its last one (zero weights)
+  uint32_t cw[kRounds][16];
   {
-    const IHE_CONSTANT uint32_t *kc = cptr(E.kcols) + T.kcol_off + 16 * (L / 32) + 4 * q;
+    const IHE_CONSTANT uint32_t *kc = cptr(E.kcols) + T.kcol_off;
 #pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) {
-      const uint32_t kbs = (uint32_t)kb < T.nkb ? (uint32_t)kb : T.nkb - 1;
-      const u32x4v c4 = *(const IHE_CONSTANT u32x4v *)(kc + 32 * kbs);
+    for (int rr = 0; rr < kRounds; ++rr) {
+      const uint32_t kb = kb_of(rr), kbs = kb < T.nkb ? kb : T.nkb - 1;
+      const IHE_CONSTANT u32x4v *c4 = (const IHE_CONSTANT u32x4v *)(kc + 32 * kbs + 16 * half_of(rr));
 #pragma unroll
-      for (int i = 0; i < 4; ++i) cw[kb][i] = col_word<ROT>(c4[i], E);
+      for (int g = 0; g < 4; ++g) {
+        const u32x4v v = c4[g];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          cw[rr][4 * g + i] = col_word<ROT>(v[i], E);
+          // unrotated columns: the lane's word joins the column word, and the
+          // column tile's offset goes in the scalar offset of the load
+          if constexpr (!ROT) cw[rr][4 * g + i] += r * 4u;
+        }
+      }
     }
   }
   const uint32_t row_base = T.slot0 * kCtWords * 4u;
   const bool full = T.nrows == rows;
 
-  uint32_t x[NKB][4];
-  auto load_tile = [&](uint32_t nt) {
+  uint32_t x[kRounds][16];
+  auto load_round = [&](int rr, uint32_t nt) {
 #pragma unroll
-    for (int kb = 0; kb < NKB; ++kb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) x[kb][i] = load_col<ROT>(in_r, cw[kb][i], col(nt) * 32 + lw);
-  };
-  auto stage_tile = [&](uint32_t buf) {
-#pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) {
-      uint32_t d[4];
-      byte_quad(x[kb], d);
-#pragma unroll
-      for (int b = 0; b < 4; ++b) s_b[buf][((kb * 4 + b) * 64 + L) * 4 + q] = d[b];
+    for (int j = 0; j < 16; ++j) {
+      if constexpr (!ROT)
+        x[rr][j] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(in_r, cw[rr][j],
+                                                                  (int)(col(nt) * 128u), 0);
+      else
+        x[rr][j] = load_col<ROT>(in_r, cw[rr][j], col(nt) * 32 + r);
     }
   };
-  auto multiply_tile = [&](uint32_t nt, uint32_t buf) {
+  auto stage_round = [&](int rr, uint32_t buf) {
+    int4v pl[4];
+    byte_planes(x[rr], pl);
+    const uint32_t at = kb_of(rr) * 4 * 64 + 32 * half_of(rr) + r;
+    if (rr < kFull || h == 0) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) s_b[buf][at + b * 64] = pl[b];
+    }
+  };
+
+#pragma unroll
+  for (int rr = 0; rr < kRounds; ++rr) load_round(rr, 0);
+#pragma unroll
+  for (int rr = 0; rr < kRounds; ++rr) stage_round(rr, 0);
+#pragma unroll
+  for (int rr = 0; rr < kRounds; ++rr) load_round(rr, NTW > 1 ? 1 : 0);
+  __syncthreads();
+  // Column tile nt: the K-blocks' MFMAs read LDS buffer nt % 2, each K-block's
+  // B operands read while the previous K-block multiplies; round rr of tile
+  // nt + 1 (in registers since the previous tile) is recoded into the other
+  // buffer after K-block (rr + 1) NKB / kRounds - 1, and its registers are
+  // refilled with tile nt + 2, so the recoding runs beside the MFMAs.
+  auto column_tile = [&](uint32_t nt) {
+    const uint32_t buf = nt & 1;
+    const uint32_t nt2 = nt + 2 < (uint32_t)NTW ? nt + 2 : NTW - 1;
     Acc c = {};
-    const int4v *sb = (const int4v *)s_b[buf];
+    int4v pl[2][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pl[0][b] = s_b[buf][b * 64 + lane];
 #pragma unroll
     for (int kb = 0; kb < NKB; ++kb) {
-      int4v pl[4];
+      if (kb + 1 < NKB) {
 #pragma unroll
-      for (int b = 0; b < 4; ++b) pl[b] = sb[(kb * 4 + b) * 64 + lane];
-      mfma_kblock(a[kb][0], a[kb][1], pl, c);
+        for (int b = 0; b < 4; ++b) pl[(kb + 1) & 1][b] = s_b[buf][((kb + 1) * 4 + b) * 64 + lane];
+      }
+      mfma_kblock(a[kb][0], a[kb][1], pl[kb & 1], c);
+#pragma unroll
+      for (int rr = 0; rr < kRounds; ++rr)
+        if (kb == (rr + 1) * NKB / kRounds - 1) {
+          stage_round(rr, buf ^ 1);
+          load_round(rr, nt2);
+        }
+      __builtin_amdgcn_sched_barrier(0);
     }
     uint32_t rbase = row_base;
     asm volatile("" : "+s"(rbase));
     finish_rows(c, E, out_r, s_bias, col(nt) * 32, r, h, 32 * wave, T.nrows, full, rbase);
-  };
-
-  load_tile(0);
-  stage_tile(0);
-  load_tile(NTW > 1 ? 1 : 0);
-  __syncthreads();
-  for (uint32_t nt = 0; nt < (uint32_t)NTW; ++nt) {
-    multiply_tile(nt, nt & 1);
-    // the loads of column tile nt + 1 (the last tile reloads itself) were
-    // issued before this tile's stores
-    stage_tile((nt + 1) & 1);
-    load_tile(nt + 2 < (uint32_t)NTW ? nt + 2 : NTW - 1);
     __syncthreads();
-  }
+  };
+  // the first column tile is peeled so that the loop is entered, like it is
+  // re-entered, with each round's loads followed by the tile's 16 stores
+  column_tile(0);
+  for (uint32_t nt = 1; nt < (uint32_t)NTW; ++nt) column_tile(nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -559,17 +594,15 @@ void launch_kk(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
   }
 }
 
+// tiles of nkb K-blocks run on the kernel with the next even NKB (padding
+// K-blocks carry zero weights)
 template <bool ROT, int NTW>
 void launch_kw(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
   const dim3 block(64 * kWaves);
-  switch (nkb) {
-    case 1: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 1>), grid, block, 0, s, A); break;
-    case 2: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 2>), grid, block, 0, s, A); break;
-    case 3: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 3>), grid, block, 0, s, A); break;
-    case 4: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 4>), grid, block, 0, s, A); break;
-    case 5: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 5>), grid, block, 0, s, A); break;
-    case 6: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 6>), grid, block, 0, s, A); break;
-    case 7: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 7>), grid, block, 0, s, A); break;
+  switch ((nkb + 1) / 2) {
+    case 1: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 2>), grid, block, 0, s, A); break;
+    case 2: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 4>), grid, block, 0, s, A); break;
+    case 3: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 6>), grid, block, 0, s, A); break;
     default: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 8>), grid, block, 0, s, A); break;
   }
 }
