@@ -10,6 +10,7 @@
 // slides a J-register window down A2 as k advances, so each key bit costs one
 // LDS read per lane plus, when the bit is set, J adds. The key bits are
 // kernel arguments, hence wave-uniform branches.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -104,100 +105,162 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_decrypt(const int32_t *
 // v_mfma_i32_32x32x32_i8 per byte plane and K-block of 32 coefficients, four
 // int32 accumulators recombined with shifts mod 2^32 (|partial| <= 2^17).
 //
-// A workgroup of 8 waves owns 32 ciphertexts x 256 samples (one 32-sample
-// column tile per wave). Per K-block the workgroup loads the 32 x 32 a-words
-// (two per lane, four K-blocks ahead of use), writes their byte planes to LDS
-// (double-buffered), and every wave reads the full A operand back; a wave's B
-// operand (its 32 columns, the K-block's 32 rows of T) is 16 consecutive bytes
-// of u[x] = S[-x] in LDS, gathered with byte funnel shifts. Workgroups of one
-// ciphertext group run on one XCD.
-constexpr uint32_t kDecWaves = 8;
-constexpr uint32_t kDecCols = 32 * kDecWaves;  // samples per workgroup
-constexpr uint32_t kDecAhead = 4;              // K-blocks of a-words in flight
+// Work items are 32 ciphertexts x 256 samples; a workgroup of 4 waves owns one
+// item at a time, each wave two 32-sample column tiles (so each A operand read
+// feeds eight MFMAs). The workgroups are persistent: each builds the key's B
+// operands once, then walks its items, which are numbered so that the sample
+// groups of one ciphertext group run together on one XCD (sharing its L2).
+//  - B: T's rows 32 ks + 16 h + jj, column j, are the 16 consecutive bytes
+//    u[N - j + 32 ks + 16 h ...] of u[N + x] = S[-x]. LDS holds 16 copies of u
+//    shifted by 0..15 bytes, so every B operand is one aligned 16-byte read;
+//    copies are 2064 bytes apart, which puts neighbouring lanes on different
+//    banks.
+//  - A: per stage of 128 coefficients the workgroup loads the 32 x 128 a-words
+//    (each load instruction reads 8 rows x 128 bytes, one stage ahead of use),
+//    writes their byte planes to LDS (double-buffered, one barrier per stage),
+//    and every wave reads its A operands back lane-linear.
+constexpr uint32_t kDecWaves = 4;
+constexpr uint32_t kDecCols = 256;              // samples per work item
+constexpr uint32_t kDecUStride = 2 * kN + 16;   // bytes between shifted copies of u
 
 struct DecryptParams {
   const int32_t *__restrict__ in;
   float *__restrict__ out;
-  uint32_t num_cts, ns, col_groups;
+  uint32_t num_cts, ns, col_groups, num_items;
 };
+
+// u[y] = S[N - y]: s[N - y] for 1 <= y <= N, -s[2N - y] for N < y < 2N, else 0
+// (key words in LDS).
+__device__ __forceinline__ uint32_t u_byte(uint32_t y, const uint32_t *kw) {
+  if (y >= 1 && y <= kN) {
+    const uint32_t m = kN - y;
+    return (kw[m / 32] >> (m % 32)) & 1u;
+  }
+  if (y > kN && y < 2 * kN) {
+    const uint32_t m = 2 * kN - y;
+    return ((kw[m / 32] >> (m % 32)) & 1u) ? 0xFFu : 0u;
+  }
+  return 0u;
+}
 
 __global__ __launch_bounds__(64 * kDecWaves) void k_decrypt_mfma(DecryptParams P, KeyBits key) {
   typedef int int4v __attribute__((ext_vector_type(4)));
   typedef int int16v __attribute__((ext_vector_type(16)));
-  typedef uint32_t u32x2v __attribute__((ext_vector_type(2)));
-  __shared__ __attribute__((aligned(16))) int8_t u[2 * kN + 64];
-  __shared__ __attribute__((aligned(16))) uint16_t planes[2][4][32][16];
-  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  __shared__ __attribute__((aligned(16))) uint8_t U[16 * kDecUStride];
+  // byte planes of one stage: [buffer][kb][plane][k half][row][4 dwords]
+  __shared__ __attribute__((aligned(16))) uint32_t planes[2][4][4][2][32][4];
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
   const uint32_t h = lane / 32, c = lane % 32;
-  // XCD-aware order: consecutive logical blocks (the column groups of one
-  // ciphertext group) share an XCD and its L2
-  const uint32_t n = gridDim.x, per = n / 8;
-  const uint32_t blk = blockIdx.x < 8 * per ? (blockIdx.x % 8) * per + blockIdx.x / 8 : blockIdx.x;
-  const uint32_t ct0 = (blk / P.col_groups) * 32;
-  const uint32_t j0 = (blk % P.col_groups) * kDecCols + wave * 32;
-  // u[N + x] = S[-x], x in (-N, N); zero padding around
-  for (uint32_t t = threadIdx.x; t < 2 * kN + 64; t += 64 * kDecWaves) {
-    const int32_t m = (int32_t)kN - (int32_t)t;  // S index
-    int8_t v = 0;
-    if (m >= 0 && m < (int32_t)kN) v = (int8_t)((key.w[m / 32] >> (m % 32)) & 1u);
-    else if (m < 0 && m > -(int32_t)kN) v = (int8_t)-(int8_t)((key.w[(m + kN) / 32] >> ((m + kN) % 32)) & 1u);
-    u[t] = v;
+  __shared__ uint32_t kw[kN / 32];
+  if (threadIdx.x < kN / 32) kw[threadIdx.x] = key.w[threadIdx.x];
+  __syncthreads();
+  // U[s * kDecUStride + y] = u[y + s]
+  for (uint32_t t = threadIdx.x; t < 16 * (2 * kN / 4); t += 64 * kDecWaves) {
+    const uint32_t sh = t / (2 * kN / 4), d = t % (2 * kN / 4);
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) v |= u_byte(4 * d + sh + i, kw) << (8 * i);
+    *(uint32_t *)(U + sh * kDecUStride + 4 * d) = v;
   }
-  // this thread's share of each K-block: ciphertext ct0 + (tid % 32), words
-  // 2 * (tid / 32) .. +1 of the block's 32 coefficients
-  const uint32_t row = threadIdx.x % 32, pair = threadIdx.x / 32;
-  const uint32_t my_ct = min(ct0 + row, P.num_cts - 1);
-  const int32_t *arow = P.in + (size_t)my_ct * kCtWords + 2 * pair;
-  const bool active = j0 < P.ns;
-  int16v acc[4] = {};
-  u32x2v x[kDecAhead];
+  __syncthreads();
+  // items of XCD x (blocks x, x + 8, ...): a contiguous range, walked in order
+  const uint32_t xcd = blockIdx.x % 8, per_xcd = gridDim.x / 8, slot = blockIdx.x / 8;
+  const uint32_t chunk = (P.num_items + 7) / 8;
+  const uint32_t it_end = min((xcd + 1) * chunk, P.num_items);
+  // loader role: row r, dword group cq of each 32-coefficient block of a stage
+  const uint32_t r = threadIdx.x / 8, cq = threadIdx.x % 8;
+  auto row_of = [&](uint32_t item) {
+    const uint32_t ct = min((item / P.col_groups) * 32 + r, P.num_cts - 1);
+    return P.in + (size_t)ct * kCtWords + 4 * cq;
+  };
+  int4v xr[4];
+  uint32_t item = xcd * chunk + slot;
+  if (item < it_end) {
+    const int32_t *arow = row_of(item);
 #pragma unroll
-  for (uint32_t q = 0; q < kDecAhead; ++q) x[q] = *(const u32x2v *)(arow + 32 * q);
-  for (uint32_t kb = 0; kb < kN / 32; kb += kDecAhead) {
+    for (int i = 0; i < 4; ++i) xr[i] = *(const int4v *)(arow + 32 * i);
+  }
+  for (; item < it_end; item += per_xcd) {
+    const uint32_t ct0 = (item / P.col_groups) * 32;
+    const uint32_t jw = (item % P.col_groups) * kDecCols + wave * 64;
+    const bool act0 = jw < P.ns, act1 = jw + 32 < P.ns;
+    const int32_t *arow = row_of(item);
+    const bool has_next = item + per_xcd < it_end;
+    const int32_t *nrow = row_of(has_next ? item + per_xcd : item);
+    const uint32_t jc0 = min(jw + c, P.ns - 1), jc1 = min(jw + 32 + c, P.ns - 1);
+    uint32_t bw[16];
+    int16v acc[2][4] = {};
+    for (uint32_t st = 0; st < kN / 128; ++st) {
+      const uint32_t buf = st & 1;
 #pragma unroll
-    for (uint32_t q = 0; q < kDecAhead; ++q) {
-      const uint32_t ks = kb + q, buf = ks & 1;
-      {
-        const uint32_t y0 = (x[q][0] + 0x80808080u) ^ 0x80808080u;
-        const uint32_t y1 = (x[q][1] + 0x80808080u) ^ 0x80808080u;
-        const uint32_t t0 = __builtin_amdgcn_perm(y1, y0, 0x05010400u);  // planes 0, 1
-        const uint32_t t1 = __builtin_amdgcn_perm(y1, y0, 0x07030602u);  // planes 2, 3
-        planes[buf][0][row][pair] = (uint16_t)t0;
-        planes[buf][1][row][pair] = (uint16_t)(t0 >> 16);
-        planes[buf][2][row][pair] = (uint16_t)t1;
-        planes[buf][3][row][pair] = (uint16_t)(t1 >> 16);
+      for (int i = 0; i < 4; ++i) {
+        uint32_t y[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y[q] = ((uint32_t)xr[i][q] + 0x80808080u) ^ 0x80808080u;
+        const uint32_t t0 = __builtin_amdgcn_perm(y[1], y[0], 0x05010400u);
+        const uint32_t t1 = __builtin_amdgcn_perm(y[1], y[0], 0x07030602u);
+        const uint32_t t2 = __builtin_amdgcn_perm(y[3], y[2], 0x05010400u);
+        const uint32_t t3 = __builtin_amdgcn_perm(y[3], y[2], 0x07030602u);
+        planes[buf][i][0][cq / 4][r][cq % 4] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+        planes[buf][i][1][cq / 4][r][cq % 4] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+        planes[buf][i][2][cq / 4][r][cq % 4] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+        planes[buf][i][3][cq / 4][r][cq % 4] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
       }
-      if (ks + kDecAhead < kN / 32) x[q] = *(const u32x2v *)(arow + 32 * (ks + kDecAhead));
+      if (st + 1 < kN / 128) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xr[i] = *(const int4v *)(arow + 128 * (st + 1) + 32 * i);
+      } else {
+        // the first column tile's b words, then the next item's first stage:
+        // both arrive while this stage multiplies
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const uint32_t ct = min(ct0 + 8 * (v / 4) + 4 * h + (v % 4), P.num_cts - 1);
+          bw[v] = (uint32_t)P.in[(size_t)ct * kCtWords + kN + jc0];
+        }
+        if (has_next) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) xr[i] = *(const int4v *)(nrow + 32 * i);
+        }
+      }
       __syncthreads();
-      if (active) {
-        // B: T rows 32*ks + 16*h + jj, column j0 + c: u[N - D - c + 16h + jj],
-        // D = j0 - 32*ks (start in [1, 2N - 16])
-        const uint32_t start = kN - j0 + 32 * ks - c + 16 * h;
-        const uint32_t base = start & ~3u, sh = start & 3u;
-        uint32_t w[5];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) w[i] = *(const uint32_t *)(u + base + 4 * i);
-        int4v b;
+      for (uint32_t kb = 0; kb < 4; ++kb) {
+        const uint32_t ks = 4 * st + kb;
+        int4v a[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) b[i] = (int)__builtin_amdgcn_alignbyte(w[i + 1], w[i], sh);
+        for (int p = 0; p < 4; ++p) a[p] = *(const int4v *)&planes[buf][kb][p][h][c][0];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          const int4v a = *(const int4v *)&planes[buf][p][c][8 * h];
-          acc[p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc[p], 0, 0, 0);
+        for (int t = 0; t < 2; ++t) {
+          if (t == 0 ? !act0 : !act1) continue;
+          const uint32_t start = kN - (jw + 32 * t) + 32 * ks - c + 16 * h;  // in [1, 2N - 16]
+          const uint32_t sh = start & 15u;
+          const int4v bop = *(const int4v *)(U + sh * kDecUStride + (start - sh));
+#pragma unroll
+          for (int p = 0; p < 4; ++p)
+            acc[t][p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[p], bop, acc[t][p], 0, 0, 0);
         }
       }
     }
-  }
-  if (!active) return;
-  const uint32_t j = j0 + c;
+    // phase = b - a*s for the two column tiles
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const uint32_t ct = ct0 + 8 * (v / 4) + 4 * h + (v % 4);
-    const uint32_t as = ((((((uint32_t)acc[3][v] << 8) + (uint32_t)acc[2][v]) << 8) +
-                          (uint32_t)acc[1][v]) << 8) + (uint32_t)acc[0][v];
-    if (ct < P.num_cts && j < P.ns) {
-      const int32_t phase = (int32_t)((uint32_t)P.in[(size_t)ct * kCtWords + kN + j] - as);
-      P.out[(size_t)ct * P.ns + j] = (float)phase * 0x1p-32f;
+    for (int t = 0; t < 2; ++t) {
+      const uint32_t j = jw + 32 * t + c;
+      if (t == 0 ? !act0 : !act1) continue;
+      if (t == 1) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const uint32_t ct = min(ct0 + 8 * (v / 4) + 4 * h + (v % 4), P.num_cts - 1);
+          bw[v] = (uint32_t)P.in[(size_t)ct * kCtWords + kN + jc1];
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const uint32_t ct = ct0 + 8 * (v / 4) + 4 * h + (v % 4);
+        const uint32_t as = ((((((uint32_t)acc[t][3][v] << 8) + (uint32_t)acc[t][2][v]) << 8) +
+                              (uint32_t)acc[t][1][v]) << 8) + (uint32_t)acc[t][0][v];
+        if (ct < P.num_cts && j < P.ns)
+          P.out[(size_t)ct * P.ns + j] = (float)(int32_t)(bw[v] - as) * 0x1p-32f;
+      }
     }
   }
 }
@@ -382,9 +445,14 @@ int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const in
   ScopedTimer t(ctx, IHE_TIMER_DECRYPT);
   const char *force = std::getenv("IHE_DECRYPT_KERNEL");
   if (!(force && std::strcmp(force, "valu") == 0)) {
-    DecryptParams P{d_in, d_out, num_cts, num_samples, (num_samples + kDecCols - 1) / kDecCols};
-    const dim3 grid(((num_cts + 31) / 32) * P.col_groups);
-    hipLaunchKernelGGL(k_decrypt_mfma, grid, dim3(64 * kDecWaves), 0, ctx->stream, P, key->bits);
+    const uint32_t col_groups = (num_samples + kDecCols - 1) / kDecCols;
+    DecryptParams P{d_in, d_out, num_cts, num_samples, col_groups,
+                    ((num_cts + 31) / 32) * col_groups};
+    // persistent: two workgroups per CU, a multiple of the 8 XCDs
+    const uint32_t blocks = std::min<uint32_t>(((uint32_t)ctx->num_cus * 2 + 7) / 8 * 8,
+                                               (P.num_items + 7) / 8 * 8);
+    hipLaunchKernelGGL(k_decrypt_mfma, dim3(blocks), dim3(64 * kDecWaves), 0, ctx->stream, P,
+                       key->bits);
     IHE_HIP(hipGetLastError());
     return IHE_OK;
   }
