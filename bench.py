@@ -300,16 +300,32 @@ def cli_pipeline(workload, log_fn, workdir=None):
     return {"workload": workload, "figures": figures, "file_bytes": sizes}
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(eng, d_ct, c, rp, ib, w, d_out, threads):
+    """The oracle's restatement of the reference's cloud loop, timed at `threads`
+    and at the README's 4 threads; the GPU rows must equal the CPU rows."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     ct = d_ct.numpy()
     var = np.full(c.num_in_cts, 2.0 ** -50)
     out, _, secs = oracle.cpu_baseline_cloud(ct, var, rp, ib, w, *c.layout, threads)
-    # the GPU rows must equal the CPU rows
     gpu = d_out.numpy()
     same = bool((gpu == out).all())
-    return secs, same
+    del out
+    secs4 = None
+    if threads != 4:
+        _, _, secs4 = oracle.cpu_baseline_cloud(ct, var, rp, ib, w, *c.layout, 4)
+    return secs, same, secs4
 
 
 def main(argv=None):
@@ -431,6 +447,32 @@ def main(argv=None):
             f"({ceiling['GB_per_s']} GB/s); cloud kernel at {ceiling['cloud_kernel_frac']:.1%} "
             f"of it")
 
+        # the step with the host hand-over of the C++ boundary: inputs copied from
+        # pinned host memory, outputs back to it (ciphertext words and variances)
+        from idash2019_2_amd import _abi as abi
+        L = abi.lib()
+        h_in = eng.pinned((C, 2, 1024), np.int32)
+        h_out = eng.pinned((c.num_outputs, 2, 1024), np.int32)
+        h_iv, h_ov = eng.pinned((C,), np.float64), eng.pinned((c.num_outputs,), np.float64)
+        h_in.array[...] = d_ct.numpy()
+        h_iv.array[...] = 2.0 ** -50
+        pcie = []
+        for _ in range(3):
+            eng.sync()
+            t0 = time.perf_counter()
+            abi.check(L.ihe_memcpy_h2d_async(eng.ctx, d_ct.ptr, h_in.ptr, d_ct.nbytes))
+            abi.check(L.ihe_memcpy_h2d_async(eng.ctx, d_var.ptr, h_iv.ptr, d_var.nbytes))
+            eng.cloud(band, d_ct, d_var, d_out, d_ov)
+            abi.check(L.ihe_memcpy_d2h_async(eng.ctx, h_out.ptr, d_out.ptr, d_out.nbytes))
+            abi.check(L.ihe_memcpy_d2h_async(eng.ctx, h_ov.ptr, d_ov.ptr, d_ov.nbytes))
+            eng.sync()
+            pcie.append(time.perf_counter() - t0)
+        for h in (h_in, h_out, h_iv, h_ov):
+            h.free()
+        pcie_s = min(pcie)
+        log(f"step with host copies (pinned, {(d_ct.nbytes + d_out.nbytes) / 1e9:.2f} GB): "
+            f"{pcie_s * 1e3:.1f} ms")
+
         # encrypt and decrypt kernels on the same workload (outside the timed steps)
         eng.timing(True)
         eng.timing_reset()
@@ -449,15 +491,19 @@ def main(argv=None):
         if not args.no_cpu_baseline:
             threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) \
                 or min(16, os.cpu_count() or 1)
-            secs, same = cpu_baseline(eng, d_ct, c, rp, ib, w, d_out, threads)
-            log(f"cpu baseline: {secs:.3f} s with {threads} threads; GPU output "
-                f"{'equals' if same else 'DIFFERS FROM'} the CPU output")
+            secs, same, secs4 = cpu_baseline(eng, d_ct, c, rp, ib, w, d_out, threads)
+            log(f"cpu baseline: {secs:.3f} s with {threads} threads"
+                + (f", {secs4:.3f} s with 4" if secs4 else "")
+                + f"; GPU output {'equals' if same else 'DIFFERS FROM'} the CPU output")
             if not same:
                 raise SystemExit("GPU output differs from the CPU baseline")
             cpu = {"value": round(c.num_targets * ns / secs, 1), "unit": "imputed target SNPs/s",
                    "cores": threads, "kind": "port",
                    "sample": f"one full cloud step of the same workload ({c.num_targets} targets "
-                             f"x {ns} samples), compute only, as eval/cloud.cpp:16-18 times it"}
+                             f"x {ns} samples), compute only, as eval/cloud.cpp:16-18 times it",
+                   "cpu_model": cpu_model()}
+            if secs4:
+                cpu["value_at_4_threads"] = round(c.num_targets * ns / secs4, 1)
         band.free()
         key.free()
         forms = None
@@ -477,6 +523,8 @@ def main(argv=None):
                    "algorithmic_bytes": alg_bytes,
                    "target_positions_per_s": round(c.num_targets / step_s, 1),
                    "cloud_kernel_ms": round(avg_kernel_s * 1e3, 4),
+                   "step_with_host_copies_ms": round(pcie_s * 1e3, 3),
+                   "value_with_host_copies": round(c.num_targets * ns / pcie_s, 1),
                    "variance_kernel_ms": round(var_ms / kern_n, 4),
                    "encrypt_kernel_ms": round(enc_ms / enc_n, 4),
                    "decrypt_kernel_ms": round(dec_ms / dec_n, 4)},

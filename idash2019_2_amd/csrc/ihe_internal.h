@@ -46,6 +46,7 @@ struct ihe_ctx {
   bool timing = false;
   ihe::TimerSlot timers[IHE_TIMER_COUNT];
   uint64_t *d_noise_table = nullptr;  // encrypt noise CDF, uploaded at context creation
+  uint32_t *d_noise_lut = nullptr;    // its bucket index (same allocation, after the table)
 };
 
 struct ihe_key {
@@ -67,5 +68,10 @@ struct ScopedTimer {
 int check_ctx(ihe_ctx *ctx);
 int make_current(ihe_ctx *ctx);
 void build_noise_table(uint64_t *table, uint32_t len);
+// Bucket index of the CDF table: for m in [b 2^51, (b + 1) 2^51), the count of
+// table entries <= m (among the first len - 1) lies in [lo, lo + span],
+// lut[b] = lo | span << 16.
+constexpr uint32_t kNoiseLutBits = 12;
+void build_noise_lut(const uint64_t *table, uint32_t len, uint32_t *lut);
 
 }  // namespace ihe

@@ -95,8 +95,70 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_decrypt(const int32_t *
   }
 }
 
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t r[4]) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    // one 32 x 32 -> 64-bit product per multiplier
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  r[0] = c0;
+  r[1] = c1;
+  r[2] = c2;
+  r[3] = c3;
+}
+
+// |e| = smallest k with (u >> 1) < table[k]; sign from bit 0. The table is
+// non-decreasing and table[LEN - 1] = 2^63 exceeds every m, so k is the count
+// of entries <= m among the first LEN - 1, found in a fixed 11 steps (no
+// data-dependent loop, so the samples of a lane interleave).
+__device__ __forceinline__ int32_t noise_sample(uint64_t u, const uint64_t *__restrict__ table) {
+  static_assert(IHE_NOISE_TABLE_LEN - 1 <= 2047, "11 steps cover the table");
+  const uint64_t m = u >> 1;
+  uint32_t k = 0;
+#pragma unroll
+  for (uint32_t step = 1024; step > 0; step >>= 1) {
+    const uint32_t nk = k + step;
+    const uint64_t tv = table[min(nk, IHE_NOISE_TABLE_LEN - 1) - 1];
+    k = nk <= IHE_NOISE_TABLE_LEN - 1 && tv <= m ? nk : k;
+  }
+  return (u & 1) ? -(int32_t)k : (int32_t)k;
+}
+
+// The same sample, the search started from the bucket index (build_noise_lut):
+// three steps where the bucket spans at most 7 entries (all but the tails
+// beyond about 3.5 sigma), the full eleven from the bucket's low end otherwise.
+__device__ __forceinline__ int32_t noise_sample_lut(uint64_t u, const uint64_t *table,
+                                                    const uint32_t *lut) {
+  const uint64_t m = u >> 1;
+  const uint32_t e = lut[m >> (63 - kNoiseLutBits)];
+  uint32_t k = e & 0xFFFFu;
+  auto step_to = [&](uint32_t step) {
+    const uint32_t nk = k + step;
+    const uint64_t tv = table[min(nk, IHE_NOISE_TABLE_LEN - 1) - 1];
+    k = nk <= IHE_NOISE_TABLE_LEN - 1 && tv <= m ? nk : k;
+  };
+  if ((e >> 16) <= 7) {
+#pragma unroll
+    for (uint32_t step = 4; step > 0; step >>= 1) step_to(step);
+  } else {
+#pragma unroll
+    for (uint32_t step = 1024; step > 0; step >>= 1) step_to(step);
+  }
+  return (u & 1) ? -(int32_t)k : (int32_t)k;
+}
+
 // ---------------------------------------------------------------------------
-// Decryption on int8 MFMA. For every ciphertext c and sample j < NS,
+// Decryption and encryption on int8 MFMA. For every ciphertext c and word j,
 //   (a_c * s)[j] = sum_i a_c[i] T[i][j],  T[i][j] = S[j - i],
 // with S[m] = s[m] for m >= 0 and -s[m + N] for m < 0 (the negacyclic wrap),
 // so decryption is the product of the ciphertexts' a-rows with one Toeplitz
@@ -119,15 +181,20 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_decrypt(const int32_t *
 //    (each load instruction reads 8 rows x 128 bytes, one stage ahead of use),
 //    writes their byte planes to LDS (double-buffered, one barrier per stage),
 //    and every wave reads its A operands back lane-linear.
+// Decryption (words j < NS) ends with phase = b - a*s as a float; encryption
+// (all N words; k_encrypt_ae has written a and b = e + messages) adds a*s to
+// b in place.
 constexpr uint32_t kDecWaves = 4;
 constexpr uint32_t kDecCols = 256;              // samples per work item
 constexpr uint32_t kDecUStride = 2 * kN + 16;   // bytes between shifted copies of u
 
 struct DecryptParams {
-  const int32_t *__restrict__ in;
-  float *__restrict__ out;
+  const int32_t *in;
+  float *out;     // decryption: phases [num_cts][ns]
+  int32_t *bout;  // encryption: the ciphertexts of `in` (same buffer), b halves updated
   uint32_t num_cts, ns, col_groups, num_items;
 };
+
 
 // u[y] = S[N - y]: s[N - y] for 1 <= y <= N, -s[2N - y] for N < y < 2N, else 0
 // (key words in LDS).
@@ -143,7 +210,9 @@ __device__ __forceinline__ uint32_t u_byte(uint32_t y, const uint32_t *kw) {
   return 0u;
 }
 
-__global__ __launch_bounds__(64 * kDecWaves) void k_decrypt_mfma(DecryptParams P, KeyBits key) {
+template <bool ENC>
+__global__ __launch_bounds__(64 * kDecWaves) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_ring_mfma(DecryptParams P, KeyBits key) {
   typedef int int4v __attribute__((ext_vector_type(4)));
   typedef int int16v __attribute__((ext_vector_type(16)));
   __shared__ __attribute__((aligned(16))) uint8_t U[16 * kDecUStride];
@@ -183,11 +252,13 @@ __global__ __launch_bounds__(64 * kDecWaves) void k_decrypt_mfma(DecryptParams P
   for (; item < it_end; item += per_xcd) {
     const uint32_t ct0 = (item / P.col_groups) * 32;
     const uint32_t jw = (item % P.col_groups) * kDecCols + wave * 64;
-    const bool act0 = jw < P.ns, act1 = jw + 32 < P.ns;
+    // decryption needs the words j < NS, encryption all N
+    const uint32_t lim = ENC ? kN : P.ns;
+    const bool act0 = jw < lim, act1 = jw + 32 < lim;
     const int32_t *arow = row_of(item);
     const bool has_next = item + per_xcd < it_end;
     const int32_t *nrow = row_of(has_next ? item + per_xcd : item);
-    const uint32_t jc0 = min(jw + c, P.ns - 1), jc1 = min(jw + 32 + c, P.ns - 1);
+    const uint32_t jc0 = min(jw + c, lim - 1), jc1 = min(jw + 32 + c, lim - 1);
     uint32_t bw[16];
     int16v acc[2][4] = {};
     for (uint32_t st = 0; st < kN / 128; ++st) {
@@ -241,7 +312,7 @@ __global__ __launch_bounds__(64 * kDecWaves) void k_decrypt_mfma(DecryptParams P
         }
       }
     }
-    // phase = b - a*s for the two column tiles
+    // decryption: phase = b - a*s; encryption: b += a*s in place
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const uint32_t j = jw + 32 * t + c;
@@ -258,48 +329,20 @@ __global__ __launch_bounds__(64 * kDecWaves) void k_decrypt_mfma(DecryptParams P
         const uint32_t ct = ct0 + 8 * (v / 4) + 4 * h + (v % 4);
         const uint32_t as = ((((((uint32_t)acc[t][3][v] << 8) + (uint32_t)acc[t][2][v]) << 8) +
                               (uint32_t)acc[t][1][v]) << 8) + (uint32_t)acc[t][0][v];
-        if (ct < P.num_cts && j < P.ns)
+        if (ct >= P.num_cts || j >= lim) continue;
+        if constexpr (ENC)
+          P.bout[(size_t)ct * kCtWords + kN + j] = (int32_t)(bw[v] + as);
+        else
           P.out[(size_t)ct * P.ns + j] = (float)(int32_t)(bw[v] - as) * 0x1p-32f;
       }
     }
   }
 }
 
-// Philox4x32-10 (Salmon et al., SC'11).
-__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                       uint32_t k0, uint32_t k1, uint32_t r[4]) {
-#pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-    c0 = hi1 ^ c1 ^ k0;
-    c1 = lo1;
-    c2 = hi0 ^ c3 ^ k1;
-    c3 = lo0;
-    k0 += 0x9E3779B9u;
-    k1 += 0xBB67AE85u;
-  }
-  r[0] = c0;
-  r[1] = c1;
-  r[2] = c2;
-  r[3] = c3;
-}
-
-// |e| = smallest k with (u >> 1) < table[k]; sign from bit 0.
-__device__ __forceinline__ int32_t noise_sample(uint64_t u, const uint64_t *__restrict__ table) {
-  const uint64_t m = u >> 1;
-  uint32_t lo = 0, hi = IHE_NOISE_TABLE_LEN - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (m < table[mid]) hi = mid;
-    else lo = mid + 1;
-  }
-  return (u & 1) ? -(int32_t)lo : (int32_t)lo;
-}
-
 struct EncryptParams {
   const int8_t *__restrict__ geno;
   const uint64_t *__restrict__ table;
+  const uint32_t *__restrict__ lut;
   int32_t *__restrict__ out;
   double *__restrict__ out_var;
   uint32_t num_cts, num_cols, ns, R, rs;
@@ -369,6 +412,57 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_encrypt(EncryptParams P
   if (lane == 0) P.out_var[ct] = 0x1p-50;
 }
 
+// Everything of encryption but a*s, with k_encrypt's streams and counters: a
+// (Philox stream 0, one counter per four words), b = e + messages (stream 1,
+// one counter per two words, e sampled from the CDF table), and the variance.
+// k_ring_mfma<true> then adds a*s to b. Persistent workgroups copy the CDF
+// table to LDS (its searches are scattered reads) and take ciphertexts in
+// turn, a thread four words j..j+3 of each.
+__global__ __launch_bounds__(256) void k_encrypt_ae(EncryptParams P) {
+  typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+  __shared__ uint64_t s_table[IHE_NOISE_TABLE_LEN];
+  __shared__ uint32_t s_lut[1u << kNoiseLutBits];
+  for (uint32_t t = threadIdx.x; t < IHE_NOISE_TABLE_LEN; t += 256) s_table[t] = P.table[t];
+  for (uint32_t t = threadIdx.x; t < (1u << kNoiseLutBits); t += 256) s_lut[t] = P.lut[t];
+  __syncthreads();
+  const uint32_t q = threadIdx.x;
+  for (uint32_t ct = blockIdx.x; ct < P.num_cts; ct += gridDim.x) {
+    int32_t *a = P.out + (size_t)ct * kCtWords;
+    uint32_t r[4];
+    philox(q, ct, 0, 0, P.seed_lo, P.seed_hi, r);
+    *(u32x4v *)(a + 4 * q) = u32x4v{r[0], r[1], r[2], r[3]};
+    uint32_t b[4];
+#pragma unroll
+    for (int pr = 0; pr < 2; ++pr) {
+      philox(2 * q + pr, ct, 1, 0, P.seed_lo, P.seed_hi, r);
+      b[2 * pr] = (uint32_t)noise_sample_lut((uint64_t)r[0] | ((uint64_t)r[1] << 32), s_table, s_lut);
+      b[2 * pr + 1] =
+          (uint32_t)noise_sample_lut((uint64_t)r[2] | ((uint64_t)r[3] << 32), s_table, s_lut);
+    }
+    if (P.geno) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t j = 4 * q + u;
+        const uint32_t rr = j / P.rs, sj = j - rr * P.rs;
+        if (rr < P.R && sj < P.ns) {
+          const uint32_t bidx = ct * P.R + rr;
+          if (bidx < P.num_cols) {
+            const uint32_t tag = bidx / 3, val = bidx - 3 * tag;
+            const int8_t g = P.geno[(size_t)tag * P.ns + sj];
+            if (g == 0 || g == 1 || g == 2) {
+              if ((uint32_t)g == val) b[u] += kOne;
+            } else {
+              b[u] += val == 0 ? kNan[0] : (val == 1 ? kNan[1] : kNan[2]);
+            }
+          }
+        }
+      }
+    }
+    *(u32x4v *)(a + kN + 4 * q) = u32x4v{b[0], b[1], b[2], b[3]};
+    if (q == 0) P.out_var[ct] = 0x1p-50;
+  }
+}
+
 }  // namespace ihe
 
 using namespace ihe;
@@ -415,6 +509,7 @@ int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint
   EncryptParams P;
   P.geno = d_geno;
   P.table = ctx->d_noise_table;
+  P.lut = ctx->d_noise_lut;
   P.out = d_out;
   P.out_var = d_out_var;
   P.num_cols = 3 * num_tags;
@@ -424,6 +519,23 @@ int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint
   P.num_cts = (P.num_cols + P.R - 1) / P.R;
   P.seed_lo = (uint32_t)seed;
   P.seed_hi = (uint32_t)(seed >> 32);
+  const char *force = std::getenv("IHE_ENCRYPT_KERNEL");
+  if (!(force && std::strcmp(force, "valu") == 0)) {
+    // a and b = e + messages, then b += a*s on the MFMA kernel
+    const uint32_t col_groups = kN / kDecCols;
+    DecryptParams D{d_out, nullptr, d_out, P.num_cts, NS, col_groups,
+                    ((P.num_cts + 31) / 32) * col_groups};
+    const uint32_t blocks = std::min<uint32_t>(((uint32_t)ctx->num_cus * 2 + 7) / 8 * 8,
+                                               (D.num_items + 7) / 8 * 8);
+    ScopedTimer t(ctx, IHE_TIMER_ENCRYPT);
+    static_assert(kN / 4 == 256, "a thread per four words of a ciphertext");
+    hipLaunchKernelGGL(k_encrypt_ae, dim3(std::min<uint32_t>(P.num_cts, (uint32_t)ctx->num_cus * 8)),
+                       dim3(256), 0, ctx->stream, P);
+    hipLaunchKernelGGL(k_ring_mfma<true>, dim3(blocks), dim3(64 * kDecWaves), 0, ctx->stream, D,
+                       key->bits);
+    IHE_HIP(hipGetLastError());
+    return IHE_OK;
+  }
   const dim3 grid((P.num_cts + kWavesPerBlock - 1) / kWavesPerBlock), block(64 * kWavesPerBlock);
   {
     ScopedTimer t(ctx, IHE_TIMER_ENCRYPT);
@@ -446,12 +558,12 @@ int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const in
   const char *force = std::getenv("IHE_DECRYPT_KERNEL");
   if (!(force && std::strcmp(force, "valu") == 0)) {
     const uint32_t col_groups = (num_samples + kDecCols - 1) / kDecCols;
-    DecryptParams P{d_in, d_out, num_cts, num_samples, col_groups,
+    DecryptParams P{d_in, d_out, nullptr, num_cts, num_samples, col_groups,
                     ((num_cts + 31) / 32) * col_groups};
     // persistent: two workgroups per CU, a multiple of the 8 XCDs
     const uint32_t blocks = std::min<uint32_t>(((uint32_t)ctx->num_cus * 2 + 7) / 8 * 8,
                                                (P.num_items + 7) / 8 * 8);
-    hipLaunchKernelGGL(k_decrypt_mfma, dim3(blocks), dim3(64 * kDecWaves), 0, ctx->stream, P,
+    hipLaunchKernelGGL(k_ring_mfma<false>, dim3(blocks), dim3(64 * kDecWaves), 0, ctx->stream, P,
                        key->bits);
     IHE_HIP(hipGetLastError());
     return IHE_OK;

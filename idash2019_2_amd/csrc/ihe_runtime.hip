@@ -49,6 +49,18 @@ void build_noise_table(uint64_t *table, uint32_t len) {
   table[len - 1] = (uint64_t)1 << 63;
 }
 
+void build_noise_lut(const uint64_t *table, uint32_t len, uint32_t *lut) {
+  const uint32_t shift = 63 - kNoiseLutBits;
+  auto count = [&](uint64_t m) {  // entries <= m among table[0 .. len - 2]
+    return (uint32_t)(std::upper_bound(table, table + len - 1, m) - table);
+  };
+  for (uint32_t b = 0; b < (1u << kNoiseLutBits); ++b) {
+    const uint32_t lo = count((uint64_t)b << shift);
+    const uint32_t hi = count((((uint64_t)b + 1) << shift) - 1);
+    lut[b] = lo | (hi - lo) << 16;
+  }
+}
+
 ScopedTimer::ScopedTimer(ihe_ctx *c, int s) : ctx(c), slot(s) {
   if (!ctx->timing) return;
   TimerSlot &t = ctx->timers[slot];
@@ -110,10 +122,16 @@ int ihe_ctx_create(int device, ihe_ctx **out) {
     return hip_fail(e, "hipStreamCreateWithFlags");
   }
   c->stream = c->own_stream;
-  uint64_t table[IHE_NOISE_TABLE_LEN];
-  build_noise_table(table, IHE_NOISE_TABLE_LEN);
-  e = hipMalloc(&c->d_noise_table, sizeof(table));
-  if (e == hipSuccess) e = hipMemcpy(c->d_noise_table, table, sizeof(table), hipMemcpyHostToDevice);
+  // the CDF table, then its bucket index, in one allocation
+  std::vector<uint64_t> table(IHE_NOISE_TABLE_LEN + (1u << kNoiseLutBits) / 2);
+  build_noise_table(table.data(), IHE_NOISE_TABLE_LEN);
+  build_noise_lut(table.data(), IHE_NOISE_TABLE_LEN,
+                  reinterpret_cast<uint32_t *>(table.data() + IHE_NOISE_TABLE_LEN));
+  e = hipMalloc(&c->d_noise_table, table.size() * sizeof(uint64_t));
+  if (e == hipSuccess)
+    e = hipMemcpy(c->d_noise_table, table.data(), table.size() * sizeof(uint64_t),
+                  hipMemcpyHostToDevice);
+  c->d_noise_lut = reinterpret_cast<uint32_t *>(c->d_noise_table + IHE_NOISE_TABLE_LEN);
   if (e != hipSuccess) {
     if (c->d_noise_table) (void)hipFree(c->d_noise_table);
     (void)hipStreamDestroy(c->own_stream);

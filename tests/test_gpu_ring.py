@@ -11,8 +11,36 @@ from idash2019_2_amd import synth
 pytestmark = pytest.mark.gpu
 
 
+# encrypt kernels: a halves by Philox then b on the int8-MFMA Toeplitz product
+# (default), and the one-wave-per-ciphertext VALU kernel (IHE_ENCRYPT_KERNEL=valu)
+@pytest.fixture(params=["mfma", "valu"])
+def enc_kernel(request, monkeypatch):
+    if request.param == "valu":
+        monkeypatch.setenv("IHE_ENCRYPT_KERNEL", "valu")
+    else:
+        monkeypatch.delenv("IHE_ENCRYPT_KERNEL", raising=False)
+    return request.param
+
+
+def messages(geno, ns, R, RS, C):
+    """setScore's one-hot and missing-value terms (eval/idash.cpp:642-679) as an
+    array added to b, computed here independently of the oracle."""
+    one, nan = 1 << 18, (131072, 87381, 43690)
+    add = np.zeros((C, 1024), np.int64)
+    tags = geno.shape[0]
+    for c in range(C):
+        for r in range(R):
+            bidx = c * R + r
+            if bidx >= 3 * tags:
+                break
+            tag, v = divmod(bidx, 3)
+            g = geno[tag].astype(np.int64)
+            add[c, r * RS:r * RS + ns] = np.where(g == v, one, 0) + np.where(g < 0, nan[v], 0)
+    return add
+
+
 @pytest.mark.parametrize("ns", [128, 1024, 100, 1, 333])
-def test_encrypt_matches_oracle(engine, ns):
+def test_encrypt_matches_oracle(engine, enc_kernel, ns):
     rng = np.random.default_rng(ns)
     tags = 23
     geno = rng.integers(0, 3, (tags, ns)).astype(np.int8)
@@ -28,6 +56,54 @@ def test_encrypt_matches_oracle(engine, ns):
     ref, rv = oracle.encrypt(geno, key, ns, R, 1024 // R, seed=0xDEADBEEF12345)
     out = d_out.numpy()
     assert (out == ref).all()
+    assert (d_var.numpy() == rv).all()
+    k.free()
+
+
+@pytest.mark.parametrize("ns,tags", [(1024, 150), (64, 150), (2, 40)])
+def test_encrypt_many_ciphertexts_and_extreme_keys(engine, enc_kernel, ns, tags):
+    """Several groups of 32 ciphertexts (a partial last one) with a random, an
+    all-zero and an all-one key."""
+    rng = np.random.default_rng(tags + ns)
+    geno = rng.integers(0, 3, (tags, ns)).astype(np.int8)
+    geno[rng.random((tags, ns)) < 0.02] = -1
+    R = 1024 // ns
+    C = (3 * tags + R - 1) // R
+    d_geno = engine.to_device(geno)
+    for key in (rng.integers(0, 2, 1024), np.zeros(1024), np.ones(1024)):
+        key = key.astype(np.int32)
+        k = engine.key(key)
+        d_out = engine.empty((C, 2, 1024), np.int32)
+        d_var = engine.empty((C,), np.float64)
+        engine.encrypt(k, ns, d_geno, 77 + int(key.sum()), d_out, d_var)
+        ref, rv = oracle.encrypt(geno, key, ns, R, 1024 // R, seed=77 + int(key.sum()))
+        assert (d_out.numpy() == ref).all()
+        assert (d_var.numpy() == rv).all()
+        k.free()
+
+
+def test_encrypt_without_genotypes_is_ph1(engine, enc_kernel):
+    """A null genotype pointer gives encrypt_data_ph1 alone (eval/idash.cpp:615-640):
+    the oracle's ciphertexts minus their messages."""
+    import ctypes
+    from idash2019_2_amd import _abi
+    ns, tags = 128, 30
+    R, RS = 8, 128
+    C = (3 * tags + R - 1) // R
+    rng = np.random.default_rng(3)
+    geno = rng.integers(0, 3, (tags, ns)).astype(np.int8)
+    geno[rng.random((tags, ns)) < 0.1] = -1
+    key = rng.integers(0, 2, 1024).astype(np.int32)
+    k = engine.key(key)
+    d_out = engine.empty((C, 2, 1024), np.int32)
+    d_var = engine.empty((C,), np.float64)
+    lay = _abi.Layout(ns, R, RS)
+    _abi.check(_abi.lib().ihe_encrypt(engine.ctx, k.handle, ctypes.byref(lay), tags, None, 99,
+                                      d_out.ptr, d_var.ptr))
+    ref, rv = oracle.encrypt(geno, key, ns, R, RS, seed=99)
+    want = ref.astype(np.int64)
+    want[:, 1, :] -= messages(geno, ns, R, RS, C)
+    assert (d_out.numpy() == want.astype(np.uint32).astype(np.int32)).all()
     assert (d_var.numpy() == rv).all()
     k.free()
 
