@@ -2,14 +2,18 @@
 //
 // The reference computes b = a*s + e (tLweSymEncryptZero, eval/idash.cpp:625)
 // and phase = b - a*s (tLwePhase, :714) through TFHE's floating-point FFT.
-// Here the key s is binary, so
-//   (a*s)[j] = sum_{k : s_k = 1} A2[j - k + N],   A2 = [-a[0..N), a[0..N)]
-// is a sum of signed coefficients. One wavefront owns one ciphertext: a is
-// staged in LDS as A2 (padded one word per J so lanes reading J-strided words
-// hit distinct banks), lane l owns the J consecutive outputs j = l*J + u and
-// slides a J-register window down A2 as k advances, so each key bit costs one
-// LDS read per lane plus, when the bit is set, J adds. The key bits are
-// kernel arguments, hence wave-uniform branches.
+// Here the key s is binary, and a*s is computed exactly two ways:
+//  - k_ring_mfma (the default for both directions): a product of the a-rows
+//    with the key's negacyclic Toeplitz matrix on int8 MFMA, described there;
+//    encryption first runs k_encrypt_ae for a and b = e + messages.
+//  - k_encrypt / k_decrypt<J> (IHE_ENCRYPT_KERNEL / IHE_DECRYPT_KERNEL =
+//    valu): (a*s)[j] = sum_{k : s_k = 1} A2[j - k + N], A2 = [-a[0..N), a[0..N)],
+//    a sum of signed coefficients. One wavefront owns one ciphertext: a is
+//    staged in LDS as A2 (padded one word per J so lanes reading J-strided
+//    words hit distinct banks), lane l owns the J consecutive outputs
+//    j = l*J + u and slides a J-register window down A2 as k advances, so each
+//    key bit costs one LDS read per lane plus, when the bit is set, J adds. The
+//    key bits are kernel arguments, hence wave-uniform branches.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
