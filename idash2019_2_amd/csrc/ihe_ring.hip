@@ -188,7 +188,6 @@ __device__ __forceinline__ int32_t noise_sample_lut(uint64_t u, const uint64_t *
 // Decryption (words j < NS) ends with phase = b - a*s as a float; encryption
 // (all N words; k_encrypt_ae has written a and b = e + messages) adds a*s to
 // b in place.
-constexpr uint32_t kDecWaves = 4;
 constexpr uint32_t kDecCols = 256;              // samples per work item
 constexpr uint32_t kDecUStride = 2 * kN + 16;   // bytes between shifted copies of u
 
@@ -214,11 +213,18 @@ __device__ __forceinline__ uint32_t u_byte(uint32_t y, const uint32_t *kw) {
   return 0u;
 }
 
-template <bool ENC>
-__global__ __launch_bounds__(64 * kDecWaves) __attribute__((amdgpu_waves_per_eu(2, 2)))
+// TW column tiles per wave, 8 / TW waves per workgroup: TW = 2 reads each A
+// operand into 8 MFMAs at 2 waves per SIMD, TW = 1 into 4 at 4 waves per SIMD.
+template <bool ENC, int TW>
+__global__ __launch_bounds__(64 * (8 / TW)) __attribute__((amdgpu_waves_per_eu(4 / TW, 4 / TW)))
 void k_ring_mfma(DecryptParams P, KeyBits key) {
   typedef int int4v __attribute__((ext_vector_type(4)));
   typedef int int16v __attribute__((ext_vector_type(16)));
+  constexpr uint32_t kDecWaves = 8 / TW;
+  // loader role: kLpr lanes per ciphertext row, each loading kNl 16-byte
+  // pieces kStride words apart per stage (each load instruction reads whole
+  // 128-byte lines of 64 * 16 / (16 * kLpr) rows)
+  constexpr uint32_t kLpr = 2 * kDecWaves, kNl = 32 / kLpr, kStride = 4 * kLpr;
   __shared__ __attribute__((aligned(16))) uint8_t U[16 * kDecUStride];
   // byte planes of one stage: [buffer][kb][plane][k half][row][4 dwords]
   __shared__ __attribute__((aligned(16))) uint32_t planes[2][4][4][2][32][4];
@@ -240,35 +246,38 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
   const uint32_t xcd = blockIdx.x % 8, per_xcd = gridDim.x / 8, slot = blockIdx.x / 8;
   const uint32_t chunk = (P.num_items + 7) / 8;
   const uint32_t it_end = min((xcd + 1) * chunk, P.num_items);
-  // loader role: row r, dword group cq of each 32-coefficient block of a stage
-  const uint32_t r = threadIdx.x / 8, cq = threadIdx.x % 8;
+  const uint32_t r = threadIdx.x / kLpr, cq = threadIdx.x % kLpr;
   auto row_of = [&](uint32_t item) {
     const uint32_t ct = min((item / P.col_groups) * 32 + r, P.num_cts - 1);
     return P.in + (size_t)ct * kCtWords + 4 * cq;
   };
-  int4v xr[4];
+  int4v xr[kNl];
   uint32_t item = xcd * chunk + slot;
   if (item < it_end) {
     const int32_t *arow = row_of(item);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) xr[i] = *(const int4v *)(arow + 32 * i);
+    for (uint32_t i = 0; i < kNl; ++i) xr[i] = *(const int4v *)(arow + kStride * i);
   }
   for (; item < it_end; item += per_xcd) {
     const uint32_t ct0 = (item / P.col_groups) * 32;
-    const uint32_t jw = (item % P.col_groups) * kDecCols + wave * 64;
+    const uint32_t jw = (item % P.col_groups) * kDecCols + wave * 32 * TW;
     // decryption needs the words j < NS, encryption all N
     const uint32_t lim = ENC ? kN : P.ns;
-    const bool act0 = jw < lim, act1 = jw + 32 < lim;
+    bool act[TW];
+#pragma unroll
+    for (int t = 0; t < TW; ++t) act[t] = jw + 32 * t < lim;
     const int32_t *arow = row_of(item);
     const bool has_next = item + per_xcd < it_end;
     const int32_t *nrow = row_of(has_next ? item + per_xcd : item);
     const uint32_t jc0 = min(jw + c, lim - 1), jc1 = min(jw + 32 + c, lim - 1);
     uint32_t bw[16];
-    int16v acc[2][4] = {};
+    int16v acc[TW][4] = {};
     for (uint32_t st = 0; st < kN / 128; ++st) {
       const uint32_t buf = st & 1;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (uint32_t i = 0; i < kNl; ++i) {
+        // coefficients w .. w + 3 of the stage: K-block w / 32, bytes k0 .. k0 + 3
+        const uint32_t w = kStride * i + 4 * cq, kb = w / 32, k0 = w % 32;
         uint32_t y[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) y[q] = ((uint32_t)xr[i][q] + 0x80808080u) ^ 0x80808080u;
@@ -276,14 +285,15 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
         const uint32_t t1 = __builtin_amdgcn_perm(y[1], y[0], 0x07030602u);
         const uint32_t t2 = __builtin_amdgcn_perm(y[3], y[2], 0x05010400u);
         const uint32_t t3 = __builtin_amdgcn_perm(y[3], y[2], 0x07030602u);
-        planes[buf][i][0][cq / 4][r][cq % 4] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
-        planes[buf][i][1][cq / 4][r][cq % 4] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
-        planes[buf][i][2][cq / 4][r][cq % 4] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
-        planes[buf][i][3][cq / 4][r][cq % 4] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+        planes[buf][kb][0][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+        planes[buf][kb][1][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+        planes[buf][kb][2][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+        planes[buf][kb][3][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
       }
       if (st + 1 < kN / 128) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) xr[i] = *(const int4v *)(arow + 128 * (st + 1) + 32 * i);
+        for (uint32_t i = 0; i < kNl; ++i)
+          xr[i] = *(const int4v *)(arow + 128 * (st + 1) + kStride * i);
       } else {
         // the first column tile's b words, then the next item's first stage:
         // both arrive while this stage multiplies
@@ -294,7 +304,7 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
         }
         if (has_next) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) xr[i] = *(const int4v *)(nrow + 32 * i);
+          for (uint32_t i = 0; i < kNl; ++i) xr[i] = *(const int4v *)(nrow + kStride * i);
         }
       }
       __syncthreads();
@@ -305,8 +315,8 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) a[p] = *(const int4v *)&planes[buf][kb][p][h][c][0];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          if (t == 0 ? !act0 : !act1) continue;
+        for (int t = 0; t < TW; ++t) {
+          if (!act[t]) continue;
           const uint32_t start = kN - (jw + 32 * t) + 32 * ks - c + 16 * h;  // in [1, 2N - 16]
           const uint32_t sh = start & 15u;
           const int4v bop = *(const int4v *)(U + sh * kDecUStride + (start - sh));
@@ -318,9 +328,9 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
     }
     // decryption: phase = b - a*s; encryption: b += a*s in place
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < TW; ++t) {
       const uint32_t j = jw + 32 * t + c;
-      if (t == 0 ? !act0 : !act1) continue;
+      if (!act[t]) continue;
       if (t == 1) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
@@ -467,6 +477,16 @@ __global__ __launch_bounds__(256) void k_encrypt_ae(EncryptParams P) {
   }
 }
 
+// IHE_RING_TILES = 1 or 2 column tiles per wave (default 2)
+template <bool ENC>
+void launch_ring(ihe_ctx *ctx, const DecryptParams &P, const KeyBits &key, uint32_t blocks) {
+  const char *e = std::getenv("IHE_RING_TILES");
+  if (e && std::strcmp(e, "1") == 0)
+    hipLaunchKernelGGL((k_ring_mfma<ENC, 1>), dim3(blocks), dim3(64 * 8), 0, ctx->stream, P, key);
+  else
+    hipLaunchKernelGGL((k_ring_mfma<ENC, 2>), dim3(blocks), dim3(64 * 4), 0, ctx->stream, P, key);
+}
+
 }  // namespace ihe
 
 using namespace ihe;
@@ -535,8 +555,7 @@ int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint
     static_assert(kN / 4 == 256, "a thread per four words of a ciphertext");
     hipLaunchKernelGGL(k_encrypt_ae, dim3(std::min<uint32_t>(P.num_cts, (uint32_t)ctx->num_cus * 8)),
                        dim3(256), 0, ctx->stream, P);
-    hipLaunchKernelGGL(k_ring_mfma<true>, dim3(blocks), dim3(64 * kDecWaves), 0, ctx->stream, D,
-                       key->bits);
+    launch_ring<true>(ctx, D, key->bits, blocks);
     IHE_HIP(hipGetLastError());
     return IHE_OK;
   }
@@ -567,8 +586,7 @@ int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const in
     // persistent: two workgroups per CU, a multiple of the 8 XCDs
     const uint32_t blocks = std::min<uint32_t>(((uint32_t)ctx->num_cus * 2 + 7) / 8 * 8,
                                                (P.num_items + 7) / 8 * 8);
-    hipLaunchKernelGGL(k_ring_mfma<false>, dim3(blocks), dim3(64 * kDecWaves), 0, ctx->stream, P,
-                       key->bits);
+    launch_ring<false>(ctx, P, key->bits, blocks);
     IHE_HIP(hipGetLastError());
     return IHE_OK;
   }

@@ -11,15 +11,24 @@ from idash2019_2_amd import synth
 pytestmark = pytest.mark.gpu
 
 
-# encrypt kernels: a halves by Philox then b on the int8-MFMA Toeplitz product
-# (default), and the one-wave-per-ciphertext VALU kernel (IHE_ENCRYPT_KERNEL=valu)
-@pytest.fixture(params=["mfma", "valu"])
-def enc_kernel(request, monkeypatch):
+def ring_kernel(request, monkeypatch, var):
+    """mfma: the int8-MFMA Toeplitz product (default, two column tiles per
+    wave); mfma1: the same with one column tile per wave (IHE_RING_TILES=1);
+    valu: the one-wave-per-ciphertext VALU kernel."""
+    monkeypatch.delenv("IHE_RING_TILES", raising=False)
+    monkeypatch.delenv(var, raising=False)
     if request.param == "valu":
-        monkeypatch.setenv("IHE_ENCRYPT_KERNEL", "valu")
-    else:
-        monkeypatch.delenv("IHE_ENCRYPT_KERNEL", raising=False)
+        monkeypatch.setenv(var, "valu")
+    elif request.param == "mfma1":
+        monkeypatch.setenv("IHE_RING_TILES", "1")
     return request.param
+
+
+# encrypt kernels: a halves by Philox then b on the MFMA product, or the VALU
+# kernel (IHE_ENCRYPT_KERNEL=valu)
+@pytest.fixture(params=["mfma", "mfma1", "valu"])
+def enc_kernel(request, monkeypatch):
+    return ring_kernel(request, monkeypatch, "IHE_ENCRYPT_KERNEL")
 
 
 def messages(geno, ns, R, RS, C):
@@ -108,15 +117,11 @@ def test_encrypt_without_genotypes_is_ph1(engine, enc_kernel):
     k.free()
 
 
-# decrypt kernels: the int8-MFMA Toeplitz product (default) and the VALU
-# sliding-window kernel (IHE_DECRYPT_KERNEL=valu)
-@pytest.fixture(params=["mfma", "valu"])
+# decrypt kernels: the MFMA product, or the VALU sliding-window kernel
+# (IHE_DECRYPT_KERNEL=valu)
+@pytest.fixture(params=["mfma", "mfma1", "valu"])
 def dec_kernel(request, monkeypatch):
-    if request.param == "valu":
-        monkeypatch.setenv("IHE_DECRYPT_KERNEL", "valu")
-    else:
-        monkeypatch.delenv("IHE_DECRYPT_KERNEL", raising=False)
-    return request.param
+    return ring_kernel(request, monkeypatch, "IHE_DECRYPT_KERNEL")
 
 
 @pytest.mark.parametrize("ns", [1, 16, 64, 65, 128, 200, 256, 512, 513, 1000, 1024])
