@@ -47,6 +47,7 @@ struct ihe_ctx {
   ihe::TimerSlot timers[IHE_TIMER_COUNT];
   uint64_t *d_noise_table = nullptr;  // encrypt noise CDF, uploaded at context creation
   uint32_t *d_noise_lut = nullptr;    // its bucket index (same allocation, after the table)
+  uint32_t *d_ring_queue = nullptr;   // ring kernels' work counters (same allocation, after the index)
 };
 
 struct ihe_key {
@@ -73,5 +74,8 @@ void build_noise_table(uint64_t *table, uint32_t len);
 // lut[b] = lo | span << 16.
 constexpr uint32_t kNoiseLutBits = 12;
 void build_noise_lut(const uint64_t *table, uint32_t len, uint32_t *lut);
+// Work counters of the persistent ring kernels: one per XCD, then a count of
+// finished workgroups (the last one clears all of them for the next launch).
+constexpr uint32_t kRingQueueWords = 16;
 
 }  // namespace ihe

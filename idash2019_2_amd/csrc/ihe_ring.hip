@@ -196,6 +196,9 @@ struct DecryptParams {
   float *out;     // decryption: phases [num_cts][ns]
   int32_t *bout;  // encryption: the ciphertexts of `in` (same buffer), b halves updated
   uint32_t num_cts, ns, col_groups, num_items;
+  // kRingQueueWords zero words: items are taken from per-XCD counters; null:
+  // workgroup x takes items x, x + 8 ... of its XCD's range
+  uint32_t *queue;
 };
 
 
@@ -242,23 +245,41 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
     *(uint32_t *)(U + sh * kDecUStride + 4 * d) = v;
   }
   __syncthreads();
-  // items of XCD x (blocks x, x + 8, ...): a contiguous range, walked in order
+  // items of XCD x (blocks x, x + 8, ...): a contiguous range, walked in order.
+  // From the counter, a workgroup takes the next item when it starts one, so
+  // the sample groups of a ciphertext group run side by side all the way
+  // (a fixed stride lets workgroups drift apart, and their shared a-words
+  // then leave L2 between the reads).
   const uint32_t xcd = blockIdx.x % 8, per_xcd = gridDim.x / 8, slot = blockIdx.x / 8;
   const uint32_t chunk = (P.num_items + 7) / 8;
+  const uint32_t it_beg = min(xcd * chunk, P.num_items);
   const uint32_t it_end = min((xcd + 1) * chunk, P.num_items);
+  // the item after the k-th is announced in s_next[(k + 1) % 2]
+  __shared__ uint32_t s_next[2];
   const uint32_t r = threadIdx.x / kLpr, cq = threadIdx.x % kLpr;
-  auto row_of = [&](uint32_t item) {
-    const uint32_t ct = min((item / P.col_groups) * 32 + r, P.num_cts - 1);
-    return P.in + (size_t)ct * kCtWords + 4 * cq;
+  // an item's ciphertext group starts at a wave-uniform address; a loader
+  // thread adds its row (clamped to the last ciphertext) as a byte offset
+  auto group_of = [&](uint32_t item) {
+    return (const char *)(P.in + (size_t)(item / P.col_groups) * 32 * kCtWords);
+  };
+  auto row_off = [&](uint32_t item) {
+    return (min(r, P.num_cts - 1 - (item / P.col_groups) * 32) * kCtWords + 4 * cq) * 4;
   };
   int4v xr[kNl];
-  uint32_t item = xcd * chunk + slot;
-  if (item < it_end) {
-    const int32_t *arow = row_of(item);
-#pragma unroll
-    for (uint32_t i = 0; i < kNl; ++i) xr[i] = *(const int4v *)(arow + kStride * i);
+  uint32_t item = it_beg + slot;
+  if (P.queue) {
+    if (threadIdx.x == 0) s_next[0] = it_beg + atomicAdd(&P.queue[xcd], 1u);
+    __syncthreads();
+    item = __builtin_amdgcn_readfirstlane(s_next[0]);
   }
-  for (; item < it_end; item += per_xcd) {
+  if (item < it_end) {
+    const char *g = group_of(item);
+    const uint32_t o = row_off(item);
+#pragma unroll
+    for (uint32_t i = 0; i < kNl; ++i) xr[i] = *(const int4v *)(g + o + 4 * kStride * i);
+  }
+  uint32_t next = 0;
+  for (uint32_t k = 1; item < it_end; item = next, k ^= 1) {
     const uint32_t ct0 = (item / P.col_groups) * 32;
     const uint32_t jw = (item % P.col_groups) * kDecCols + wave * 32 * TW;
     // decryption needs the words j < NS, encryption all N
@@ -266,14 +287,17 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
     bool act[TW];
 #pragma unroll
     for (int t = 0; t < TW; ++t) act[t] = jw + 32 * t < lim;
-    const int32_t *arow = row_of(item);
-    const bool has_next = item + per_xcd < it_end;
-    const int32_t *nrow = row_of(has_next ? item + per_xcd : item);
+    const char *ag = group_of(item);
+    const uint32_t ao = row_off(item);
     const uint32_t jc0 = min(jw + c, lim - 1), jc1 = min(jw + 32 + c, lim - 1);
     uint32_t bw[16];
     int16v acc[TW][4] = {};
     for (uint32_t st = 0; st < kN / 128; ++st) {
       const uint32_t buf = st & 1;
+      // the next item from the counter, announced at this stage's barrier
+      const bool grab = P.queue && st == 0 && threadIdx.x == 0;
+      uint32_t taken = 0;
+      if (grab) taken = atomicAdd(&P.queue[xcd], 1u);  // first used at the barrier
 #pragma unroll
       for (uint32_t i = 0; i < kNl; ++i) {
         // coefficients w .. w + 3 of the stage: K-block w / 32, bytes k0 .. k0 + 3
@@ -293,20 +317,24 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
       if (st + 1 < kN / 128) {
 #pragma unroll
         for (uint32_t i = 0; i < kNl; ++i)
-          xr[i] = *(const int4v *)(arow + 128 * (st + 1) + kStride * i);
+          xr[i] = *(const int4v *)(ag + ao + 4 * (128 * (st + 1) + kStride * i));
       } else {
         // the first column tile's b words, then the next item's first stage:
         // both arrive while this stage multiplies
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
           const uint32_t ct = min(ct0 + 8 * (v / 4) + 4 * h + (v % 4), P.num_cts - 1);
-          bw[v] = (uint32_t)P.in[(size_t)ct * kCtWords + kN + jc0];
+          bw[v] = *(const uint32_t *)(ag + ((ct - ct0) * kCtWords + kN + jc0) * 4);
         }
-        if (has_next) {
+        next = P.queue ? it_beg + __builtin_amdgcn_readfirstlane(s_next[k]) : item + per_xcd;
+        if (next < it_end) {
+          const char *ng = group_of(next);
+          const uint32_t no = row_off(next);
 #pragma unroll
-          for (uint32_t i = 0; i < kNl; ++i) xr[i] = *(const int4v *)(nrow + kStride * i);
+          for (uint32_t i = 0; i < kNl; ++i) xr[i] = *(const int4v *)(ng + no + 4 * kStride * i);
         }
       }
+      if (grab) s_next[k] = taken;
       __syncthreads();
 #pragma unroll
       for (uint32_t kb = 0; kb < 4; ++kb) {
@@ -327,6 +355,38 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
       }
     }
     // decryption: phase = b - a*s; encryption: b += a*s in place
+    if (ct0 + 32 <= P.num_cts && jw + 32 * TW <= lim) {
+      // whole tiles: one wave-uniform base per item, 32-bit byte offsets
+      // (row v of the tile is ciphertext ct0 + 8 (v / 4) + 4 h + v % 4)
+      const char *in_base = (const char *)(P.in + (size_t)ct0 * kCtWords);
+      char *out_base = ENC ? (char *)(P.bout + (size_t)ct0 * kCtWords)
+                           : (char *)(P.out + (size_t)ct0 * P.ns);
+      const uint32_t in_lane = (4 * h * kCtWords + kN + jw + c) * 4;
+      const uint32_t out_row = ENC ? kCtWords * 4 : P.ns * 4;
+      const uint32_t out_lane = ENC ? in_lane : (4 * h * P.ns + jw + c) * 4;
+      auto in_off = [&](int v) { return in_lane + (8 * (v / 4) + (v % 4)) * kCtWords * 4; };
+      auto out_off = [&](int v) { return out_lane + (8 * (v / 4) + (v % 4)) * out_row; };
+      uint32_t bw1[TW == 2 ? 16 : 1];
+      if constexpr (TW == 2) {
+        // the second tile's b words arrive while the first tile is stored
+#pragma unroll
+        for (int v = 0; v < 16; ++v) bw1[v] = *(const uint32_t *)(in_base + in_off(v) + 128);
+      }
+#pragma unroll
+      for (int t = 0; t < TW; ++t) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const uint32_t as = ((((((uint32_t)acc[t][3][v] << 8) + (uint32_t)acc[t][2][v]) << 8) +
+                                (uint32_t)acc[t][1][v]) << 8) + (uint32_t)acc[t][0][v];
+          const uint32_t b = t == 0 ? bw[v] : bw1[v];
+          if constexpr (ENC)
+            *(int32_t *)(out_base + out_off(v) + 128 * t) = (int32_t)(b + as);
+          else
+            *(float *)(out_base + out_off(v) + 128 * t) = (float)(int32_t)(b - as) * 0x1p-32f;
+        }
+      }
+      continue;
+    }
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
       const uint32_t j = jw + 32 * t + c;
@@ -335,7 +395,7 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
           const uint32_t ct = min(ct0 + 8 * (v / 4) + 4 * h + (v % 4), P.num_cts - 1);
-          bw[v] = (uint32_t)P.in[(size_t)ct * kCtWords + kN + jc1];
+          bw[v] = *(const uint32_t *)(ag + ((ct - ct0) * kCtWords + kN + jc1) * 4);
         }
       }
 #pragma unroll
@@ -350,6 +410,12 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
           P.out[(size_t)ct * P.ns + j] = (float)(int32_t)(bw[v] - as) * 0x1p-32f;
       }
     }
+  }
+  // every workgroup has taken its last (empty) item: the last one to finish
+  // clears the counters for the next launch
+  if (P.queue && threadIdx.x == 0) {
+    if (atomicAdd(&P.queue[8], 1u) == gridDim.x - 1)
+      for (uint32_t i = 0; i <= 8; ++i) atomicExch(&P.queue[i], 0u);
   }
 }
 
@@ -477,9 +543,13 @@ __global__ __launch_bounds__(256) void k_encrypt_ae(EncryptParams P) {
   }
 }
 
-// IHE_RING_TILES = 1 or 2 column tiles per wave (default 2)
+// IHE_RING_TILES = 1 or 2 column tiles per wave (default 2); IHE_RING_QUEUE=0
+// gives the workgroups fixed items instead of the context's counters
 template <bool ENC>
-void launch_ring(ihe_ctx *ctx, const DecryptParams &P, const KeyBits &key, uint32_t blocks) {
+void launch_ring(ihe_ctx *ctx, DecryptParams P, const KeyBits &key, uint32_t blocks) {
+  static_assert(kRingQueueWords >= 9, "eight XCD counters and a finished count");
+  const char *q = std::getenv("IHE_RING_QUEUE");
+  P.queue = (q && std::strcmp(q, "0") == 0) ? nullptr : ctx->d_ring_queue;
   const char *e = std::getenv("IHE_RING_TILES");
   if (e && std::strcmp(e, "1") == 0)
     hipLaunchKernelGGL((k_ring_mfma<ENC, 1>), dim3(blocks), dim3(64 * 8), 0, ctx->stream, P, key);
@@ -548,7 +618,7 @@ int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint
     // a and b = e + messages, then b += a*s on the MFMA kernel
     const uint32_t col_groups = kN / kDecCols;
     DecryptParams D{d_out, nullptr, d_out, P.num_cts, NS, col_groups,
-                    ((P.num_cts + 31) / 32) * col_groups};
+                    ((P.num_cts + 31) / 32) * col_groups, nullptr};
     const uint32_t blocks = std::min<uint32_t>(((uint32_t)ctx->num_cus * 2 + 7) / 8 * 8,
                                                (D.num_items + 7) / 8 * 8);
     ScopedTimer t(ctx, IHE_TIMER_ENCRYPT);
@@ -582,7 +652,7 @@ int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const in
   if (!(force && std::strcmp(force, "valu") == 0)) {
     const uint32_t col_groups = (num_samples + kDecCols - 1) / kDecCols;
     DecryptParams P{d_in, d_out, nullptr, num_cts, num_samples, col_groups,
-                    ((num_cts + 31) / 32) * col_groups};
+                    ((num_cts + 31) / 32) * col_groups, nullptr};
     // persistent: two workgroups per CU, a multiple of the 8 XCDs
     const uint32_t blocks = std::min<uint32_t>(((uint32_t)ctx->num_cus * 2 + 7) / 8 * 8,
                                                (P.num_items + 7) / 8 * 8);

@@ -122,8 +122,9 @@ int ihe_ctx_create(int device, ihe_ctx **out) {
     return hip_fail(e, "hipStreamCreateWithFlags");
   }
   c->stream = c->own_stream;
-  // the CDF table, then its bucket index, in one allocation
-  std::vector<uint64_t> table(IHE_NOISE_TABLE_LEN + (1u << kNoiseLutBits) / 2);
+  // the CDF table, its bucket index and the ring kernels' work counters (zero)
+  // in one allocation
+  std::vector<uint64_t> table(IHE_NOISE_TABLE_LEN + (1u << kNoiseLutBits) / 2 + kRingQueueWords / 2, 0);
   build_noise_table(table.data(), IHE_NOISE_TABLE_LEN);
   build_noise_lut(table.data(), IHE_NOISE_TABLE_LEN,
                   reinterpret_cast<uint32_t *>(table.data() + IHE_NOISE_TABLE_LEN));
@@ -132,6 +133,7 @@ int ihe_ctx_create(int device, ihe_ctx **out) {
     e = hipMemcpy(c->d_noise_table, table.data(), table.size() * sizeof(uint64_t),
                   hipMemcpyHostToDevice);
   c->d_noise_lut = reinterpret_cast<uint32_t *>(c->d_noise_table + IHE_NOISE_TABLE_LEN);
+  c->d_ring_queue = c->d_noise_lut + (1u << kNoiseLutBits);
   if (e != hipSuccess) {
     if (c->d_noise_table) (void)hipFree(c->d_noise_table);
     (void)hipStreamDestroy(c->own_stream);
@@ -158,6 +160,10 @@ int ihe_ctx_destroy(ihe_ctx *ctx) {
 
 int ihe_ctx_set_stream(ihe_ctx *ctx, void *hip_stream) {
   if (int rc = check_ctx(ctx)) return rc;
+  // the ring kernels' work counters serve one launch at a time: let the
+  // old stream's work end before launches go to the new one
+  IHE_HIP(hipSetDevice(ctx->device));
+  IHE_HIP(hipStreamSynchronize(ctx->stream));
   ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
   return IHE_OK;
 }

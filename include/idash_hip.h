@@ -88,7 +88,9 @@ const char *ihe_last_error(void); /* thread-local message of the last failure */
 int ihe_device_count(int *count);
 int ihe_ctx_create(int device, ihe_ctx **out);
 int ihe_ctx_destroy(ihe_ctx *ctx);
-int ihe_ctx_set_stream(ihe_ctx *ctx, void *hip_stream); /* NULL: back to the context's own */
+/* NULL: back to the context's own. Waits for the current stream's work first:
+ * a context's launches run one at a time. */
+int ihe_ctx_set_stream(ihe_ctx *ctx, void *hip_stream);
 int ihe_ctx_get_stream(ihe_ctx *ctx, void **hip_stream);
 int ihe_sync(ihe_ctx *ctx);
 

@@ -12,21 +12,24 @@ pytestmark = pytest.mark.gpu
 
 
 def ring_kernel(request, monkeypatch, var):
-    """mfma: the int8-MFMA Toeplitz product (default, two column tiles per
-    wave); mfma1: the same with one column tile per wave (IHE_RING_TILES=1);
+    """mfma: the int8-MFMA Toeplitz product (default: two column tiles per
+    wave, items from the context's counters); mfma1: one column tile per wave
+    (IHE_RING_TILES=1); mfma_fixed: fixed items per workgroup (IHE_RING_QUEUE=0);
     valu: the one-wave-per-ciphertext VALU kernel."""
-    monkeypatch.delenv("IHE_RING_TILES", raising=False)
-    monkeypatch.delenv(var, raising=False)
+    for v in ("IHE_RING_TILES", "IHE_RING_QUEUE", var):
+        monkeypatch.delenv(v, raising=False)
     if request.param == "valu":
         monkeypatch.setenv(var, "valu")
     elif request.param == "mfma1":
         monkeypatch.setenv("IHE_RING_TILES", "1")
+    elif request.param == "mfma_fixed":
+        monkeypatch.setenv("IHE_RING_QUEUE", "0")
     return request.param
 
 
 # encrypt kernels: a halves by Philox then b on the MFMA product, or the VALU
 # kernel (IHE_ENCRYPT_KERNEL=valu)
-@pytest.fixture(params=["mfma", "mfma1", "valu"])
+@pytest.fixture(params=["mfma", "mfma1", "mfma_fixed", "valu"])
 def enc_kernel(request, monkeypatch):
     return ring_kernel(request, monkeypatch, "IHE_ENCRYPT_KERNEL")
 
@@ -119,7 +122,7 @@ def test_encrypt_without_genotypes_is_ph1(engine, enc_kernel):
 
 # decrypt kernels: the MFMA product, or the VALU sliding-window kernel
 # (IHE_DECRYPT_KERNEL=valu)
-@pytest.fixture(params=["mfma", "mfma1", "valu"])
+@pytest.fixture(params=["mfma", "mfma1", "mfma_fixed", "valu"])
 def dec_kernel(request, monkeypatch):
     return ring_kernel(request, monkeypatch, "IHE_DECRYPT_KERNEL")
 
