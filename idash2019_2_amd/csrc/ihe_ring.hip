@@ -336,21 +336,31 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
       }
       if (grab) s_next[k] = taken;
       __syncthreads();
-#pragma unroll
-      for (uint32_t kb = 0; kb < 4; ++kb) {
+      // operands of K-block kb: the four A planes, the B column of each tile
+      auto operands = [&](uint32_t kb, int4v (&a)[4], int4v (&bop)[TW]) {
         const uint32_t ks = 4 * st + kb;
-        int4v a[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) a[p] = *(const int4v *)&planes[buf][kb][p][h][c][0];
 #pragma unroll
         for (int t = 0; t < TW; ++t) {
-          if (!act[t]) continue;
           const uint32_t start = kN - (jw + 32 * t) + 32 * ks - c + 16 * h;  // in [1, 2N - 16]
           const uint32_t sh = start & 15u;
-          const int4v bop = *(const int4v *)(U + sh * kDecUStride + (start - sh));
+          bop[t] = *(const int4v *)(U + sh * kDecUStride + (start - sh));
+        }
+      };
+      // a wave with any active tile multiplies all of them (a tile past NS
+      // costs only its multiplications: its B reads stay inside u), so the
+      // K-block loop has no branches and one wait per K-block
+      if (act[0]) {
 #pragma unroll
-          for (int p = 0; p < 4; ++p)
-            acc[t][p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[p], bop, acc[t][p], 0, 0, 0);
+        for (uint32_t kb = 0; kb < 4; ++kb) {
+          int4v a[4], bop[TW];
+          operands(kb, a, bop);
+#pragma unroll
+          for (int t = 0; t < TW; ++t)
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+              acc[t][p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[p], bop[t], acc[t][p], 0, 0, 0);
         }
       }
     }
