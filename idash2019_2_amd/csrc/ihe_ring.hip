@@ -279,41 +279,149 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
     for (uint32_t i = 0; i < kNl; ++i) xr[i] = *(const int4v *)(g + o + 4 * kStride * i);
   }
   uint32_t next = 0;
+  // An item's results are written in the next item's first stage, after that
+  // stage's loads are issued: the memory counter covers stores too, so waits
+  // for loads issued after the stores would also wait for the stores.
+  int16v acc[TW][4];
+  uint32_t bw[16];  // first column tile's b words, loaded in the last stage
+  bool pend = false;
+  uint32_t p_ct0 = 0, p_jw = 0;
+  // decryption needs the words j < NS, encryption all N
+  const uint32_t lim = ENC ? kN : P.ns;
+  auto whole = [&](uint32_t ct0, uint32_t jw) {
+    return ct0 + 32 <= P.num_cts && jw + 32 * TW <= lim;
+  };
+  // whole tiles: one wave-uniform base per item, 32-bit byte offsets (row v of
+  // a tile is ciphertext ct0 + 8 (v / 4) + 4 h + v % 4)
+  auto in_off = [&](uint32_t jw, int v) {
+    return (4 * h * kCtWords + kN + jw + c + (8 * (v / 4) + (v % 4)) * kCtWords) * 4;
+  };
+  auto load_bw1 = [&](uint32_t ct0, uint32_t jw, uint32_t (&bw1)[16]) {
+    const char *in_base = (const char *)(P.in + (size_t)ct0 * kCtWords);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) bw1[v] = *(const uint32_t *)(in_base + in_off(jw, v) + 128);
+  };
+  // decryption: phase = b - a*s; encryption: b += a*s in place
+  auto finish = [&](uint32_t ct0, uint32_t jw, const uint32_t (&bw1)[16]) {
+    auto as_of = [&](int t, int v) {
+      return ((((((uint32_t)acc[t][3][v] << 8) + (uint32_t)acc[t][2][v]) << 8) +
+               (uint32_t)acc[t][1][v]) << 8) + (uint32_t)acc[t][0][v];
+    };
+    if (whole(ct0, jw)) {
+      char *out_base = ENC ? (char *)(P.bout + (size_t)ct0 * kCtWords)
+                           : (char *)(P.out + (size_t)ct0 * P.ns);
+      const uint32_t out_lane = (4 * h * P.ns + jw + c) * 4;
+#pragma unroll
+      for (int t = 0; t < TW; ++t) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const uint32_t as = as_of(t, v), b = t == 0 ? bw[v] : bw1[v];
+          if constexpr (ENC)
+            *(int32_t *)(out_base + in_off(jw, v) + 128 * t) = (int32_t)(b + as);
+          else
+            *(float *)(out_base + out_lane + (8 * (v / 4) + (v % 4)) * P.ns * 4 + 128 * t) =
+                (float)(int32_t)(b - as) * 0x1p-32f;
+        }
+      }
+      return;
+    }
+    const char *ag = (const char *)(P.in + (size_t)ct0 * kCtWords);
+    const uint32_t jc1 = min(jw + 32 + c, lim - 1);
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+      const uint32_t j = jw + 32 * t + c;
+      if (jw + 32 * t >= lim) continue;
+      if (t == 1) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const uint32_t ct = min(ct0 + 8 * (v / 4) + 4 * h + (v % 4), P.num_cts - 1);
+          bw[v] = *(const uint32_t *)(ag + ((ct - ct0) * kCtWords + kN + jc1) * 4);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const uint32_t ct = ct0 + 8 * (v / 4) + 4 * h + (v % 4);
+        const uint32_t as = as_of(t, v);
+        if (ct >= P.num_cts || j >= lim) continue;
+        if constexpr (ENC)
+          P.bout[(size_t)ct * kCtWords + kN + j] = (int32_t)(bw[v] + as);
+        else
+          P.out[(size_t)ct * P.ns + j] = (float)(int32_t)(bw[v] - as) * 0x1p-32f;
+      }
+    }
+  };
+  // byte planes of the loaded stage into planes[buf]
+  auto recode = [&](uint32_t buf) {
+#pragma unroll
+    for (uint32_t i = 0; i < kNl; ++i) {
+      // coefficients w .. w + 3 of the stage: K-block w / 32, bytes k0 .. k0 + 3
+      const uint32_t w = kStride * i + 4 * cq, kb = w / 32, k0 = w % 32;
+      uint32_t y[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) y[q] = ((uint32_t)xr[i][q] + 0x80808080u) ^ 0x80808080u;
+      const uint32_t t0 = __builtin_amdgcn_perm(y[1], y[0], 0x05010400u);
+      const uint32_t t1 = __builtin_amdgcn_perm(y[1], y[0], 0x07030602u);
+      const uint32_t t2 = __builtin_amdgcn_perm(y[3], y[2], 0x05010400u);
+      const uint32_t t3 = __builtin_amdgcn_perm(y[3], y[2], 0x07030602u);
+      planes[buf][kb][0][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+      planes[buf][kb][1][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+      planes[buf][kb][2][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+      planes[buf][kb][3][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+    }
+  };
+  // stage st's K-blocks. A wave with any active tile multiplies all of them
+  // (a tile past NS costs only its multiplications: its B reads stay inside
+  // u), so the K-block loop has no branches and one wait per K-block.
+  auto multiply = [&](uint32_t st, uint32_t jw) {
+    const uint32_t buf = st & 1;
+#pragma unroll
+    for (uint32_t kb = 0; kb < 4; ++kb) {
+      const uint32_t ks = 4 * st + kb;
+      int4v a[4], bop[TW];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) a[p] = *(const int4v *)&planes[buf][kb][p][h][c][0];
+#pragma unroll
+      for (int t = 0; t < TW; ++t) {
+        const uint32_t start = kN - (jw + 32 * t) + 32 * ks - c + 16 * h;  // in [1, 2N - 16]
+        const uint32_t sh = start & 15u;
+        bop[t] = *(const int4v *)(U + sh * kDecUStride + (start - sh));
+      }
+#pragma unroll
+      for (int t = 0; t < TW; ++t)
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          acc[t][p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[p], bop[t], acc[t][p], 0, 0, 0);
+    }
+  };
   for (uint32_t k = 1; item < it_end; item = next, k ^= 1) {
     const uint32_t ct0 = (item / P.col_groups) * 32;
     const uint32_t jw = (item % P.col_groups) * kDecCols + wave * 32 * TW;
-    // decryption needs the words j < NS, encryption all N
-    const uint32_t lim = ENC ? kN : P.ns;
-    bool act[TW];
-#pragma unroll
-    for (int t = 0; t < TW; ++t) act[t] = jw + 32 * t < lim;
     const char *ag = group_of(item);
     const uint32_t ao = row_off(item);
-    const uint32_t jc0 = min(jw + c, lim - 1), jc1 = min(jw + 32 + c, lim - 1);
-    uint32_t bw[16];
-    int16v acc[TW][4] = {};
-    for (uint32_t st = 0; st < kN / 128; ++st) {
-      const uint32_t buf = st & 1;
-      // the next item from the counter, announced at this stage's barrier
-      const bool grab = P.queue && st == 0 && threadIdx.x == 0;
+    const uint32_t jc0 = min(jw + c, lim - 1);
+    {
+      // first stage: take the next item from the counter (announced at this
+      // stage's barrier), write the previous item's results after issuing
+      // this stage's loads
+      const bool grab = P.queue && threadIdx.x == 0;
       uint32_t taken = 0;
       if (grab) taken = atomicAdd(&P.queue[xcd], 1u);  // first used at the barrier
+      recode(0);
+      uint32_t bw1[16];
+      if (TW == 2 && pend && whole(p_ct0, p_jw)) load_bw1(p_ct0, p_jw, bw1);
 #pragma unroll
-      for (uint32_t i = 0; i < kNl; ++i) {
-        // coefficients w .. w + 3 of the stage: K-block w / 32, bytes k0 .. k0 + 3
-        const uint32_t w = kStride * i + 4 * cq, kb = w / 32, k0 = w % 32;
-        uint32_t y[4];
+      for (uint32_t i = 0; i < kNl; ++i) xr[i] = *(const int4v *)(ag + ao + 4 * (128 + kStride * i));
+      if (pend) finish(p_ct0, p_jw, bw1);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) y[q] = ((uint32_t)xr[i][q] + 0x80808080u) ^ 0x80808080u;
-        const uint32_t t0 = __builtin_amdgcn_perm(y[1], y[0], 0x05010400u);
-        const uint32_t t1 = __builtin_amdgcn_perm(y[1], y[0], 0x07030602u);
-        const uint32_t t2 = __builtin_amdgcn_perm(y[3], y[2], 0x05010400u);
-        const uint32_t t3 = __builtin_amdgcn_perm(y[3], y[2], 0x07030602u);
-        planes[buf][kb][0][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
-        planes[buf][kb][1][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
-        planes[buf][kb][2][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
-        planes[buf][kb][3][k0 / 16][r][(k0 % 16) / 4] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
-      }
+      for (int t = 0; t < TW; ++t)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) acc[t][p] = int16v{};
+      if (grab) s_next[k] = taken;
+      __syncthreads();
+      if (jw < lim) multiply(0, jw);
+    }
+    for (uint32_t st = 1; st < kN / 128; ++st) {
+      recode(st & 1);
       if (st + 1 < kN / 128) {
 #pragma unroll
         for (uint32_t i = 0; i < kNl; ++i)
@@ -334,92 +442,17 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
           for (uint32_t i = 0; i < kNl; ++i) xr[i] = *(const int4v *)(ng + no + 4 * kStride * i);
         }
       }
-      if (grab) s_next[k] = taken;
       __syncthreads();
-      // operands of K-block kb: the four A planes, the B column of each tile
-      auto operands = [&](uint32_t kb, int4v (&a)[4], int4v (&bop)[TW]) {
-        const uint32_t ks = 4 * st + kb;
-#pragma unroll
-        for (int p = 0; p < 4; ++p) a[p] = *(const int4v *)&planes[buf][kb][p][h][c][0];
-#pragma unroll
-        for (int t = 0; t < TW; ++t) {
-          const uint32_t start = kN - (jw + 32 * t) + 32 * ks - c + 16 * h;  // in [1, 2N - 16]
-          const uint32_t sh = start & 15u;
-          bop[t] = *(const int4v *)(U + sh * kDecUStride + (start - sh));
-        }
-      };
-      // a wave with any active tile multiplies all of them (a tile past NS
-      // costs only its multiplications: its B reads stay inside u), so the
-      // K-block loop has no branches and one wait per K-block
-      if (act[0]) {
-#pragma unroll
-        for (uint32_t kb = 0; kb < 4; ++kb) {
-          int4v a[4], bop[TW];
-          operands(kb, a, bop);
-#pragma unroll
-          for (int t = 0; t < TW; ++t)
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-              acc[t][p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[p], bop[t], acc[t][p], 0, 0, 0);
-        }
-      }
+      if (jw < lim) multiply(st, jw);
     }
-    // decryption: phase = b - a*s; encryption: b += a*s in place
-    if (ct0 + 32 <= P.num_cts && jw + 32 * TW <= lim) {
-      // whole tiles: one wave-uniform base per item, 32-bit byte offsets
-      // (row v of the tile is ciphertext ct0 + 8 (v / 4) + 4 h + v % 4)
-      const char *in_base = (const char *)(P.in + (size_t)ct0 * kCtWords);
-      char *out_base = ENC ? (char *)(P.bout + (size_t)ct0 * kCtWords)
-                           : (char *)(P.out + (size_t)ct0 * P.ns);
-      const uint32_t in_lane = (4 * h * kCtWords + kN + jw + c) * 4;
-      const uint32_t out_row = ENC ? kCtWords * 4 : P.ns * 4;
-      const uint32_t out_lane = ENC ? in_lane : (4 * h * P.ns + jw + c) * 4;
-      auto in_off = [&](int v) { return in_lane + (8 * (v / 4) + (v % 4)) * kCtWords * 4; };
-      auto out_off = [&](int v) { return out_lane + (8 * (v / 4) + (v % 4)) * out_row; };
-      uint32_t bw1[TW == 2 ? 16 : 1];
-      if constexpr (TW == 2) {
-        // the second tile's b words arrive while the first tile is stored
-#pragma unroll
-        for (int v = 0; v < 16; ++v) bw1[v] = *(const uint32_t *)(in_base + in_off(v) + 128);
-      }
-#pragma unroll
-      for (int t = 0; t < TW; ++t) {
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const uint32_t as = ((((((uint32_t)acc[t][3][v] << 8) + (uint32_t)acc[t][2][v]) << 8) +
-                                (uint32_t)acc[t][1][v]) << 8) + (uint32_t)acc[t][0][v];
-          const uint32_t b = t == 0 ? bw[v] : bw1[v];
-          if constexpr (ENC)
-            *(int32_t *)(out_base + out_off(v) + 128 * t) = (int32_t)(b + as);
-          else
-            *(float *)(out_base + out_off(v) + 128 * t) = (float)(int32_t)(b - as) * 0x1p-32f;
-        }
-      }
-      continue;
-    }
-#pragma unroll
-    for (int t = 0; t < TW; ++t) {
-      const uint32_t j = jw + 32 * t + c;
-      if (!act[t]) continue;
-      if (t == 1) {
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const uint32_t ct = min(ct0 + 8 * (v / 4) + 4 * h + (v % 4), P.num_cts - 1);
-          bw[v] = *(const uint32_t *)(ag + ((ct - ct0) * kCtWords + kN + jc1) * 4);
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const uint32_t ct = ct0 + 8 * (v / 4) + 4 * h + (v % 4);
-        const uint32_t as = ((((((uint32_t)acc[t][3][v] << 8) + (uint32_t)acc[t][2][v]) << 8) +
-                              (uint32_t)acc[t][1][v]) << 8) + (uint32_t)acc[t][0][v];
-        if (ct >= P.num_cts || j >= lim) continue;
-        if constexpr (ENC)
-          P.bout[(size_t)ct * kCtWords + kN + j] = (int32_t)(bw[v] + as);
-        else
-          P.out[(size_t)ct * P.ns + j] = (float)(int32_t)(bw[v] - as) * 0x1p-32f;
-      }
-    }
+    pend = true;
+    p_ct0 = ct0;
+    p_jw = jw;
+  }
+  if (pend) {
+    uint32_t bw1[16];
+    if (TW == 2 && whole(p_ct0, p_jw)) load_bw1(p_ct0, p_jw, bw1);
+    finish(p_ct0, p_jw, bw1);
   }
   // every workgroup has taken its last (empty) item: the last one to finish
   // clears the counters for the next launch
