@@ -174,8 +174,8 @@ __device__ __forceinline__ int32_t noise_sample_lut(uint64_t u, const uint64_t *
 // Work items are 32 ciphertexts x 256 samples; a workgroup of 4 waves owns one
 // item at a time, each wave two 32-sample column tiles (so each A operand read
 // feeds eight MFMAs). The workgroups are persistent: each builds the key's B
-// operands once, then walks its items, which are numbered so that the sample
-// groups of one ciphertext group run together on one XCD (sharing its L2).
+// operands once, then takes items from its XCD's counter, so the sample groups
+// of one ciphertext group run side by side on one XCD (sharing its L2).
 //  - B: T's rows 32 ks + 16 h + jj, column j, are the 16 consecutive bytes
 //    u[N - j + 32 ks + 16 h ...] of u[N + x] = S[-x]. LDS holds 16 copies of u
 //    shifted by 0..15 bytes, so every B operand is one aligned 16-byte read;
@@ -187,7 +187,7 @@ __device__ __forceinline__ int32_t noise_sample_lut(uint64_t u, const uint64_t *
 //    and every wave reads its A operands back lane-linear.
 // Decryption (words j < NS) ends with phase = b - a*s as a float; encryption
 // (all N words; k_encrypt_ae has written a and b = e + messages) adds a*s to
-// b in place.
+// b in place. An item's results are written in the next item's first stage.
 constexpr uint32_t kDecCols = 256;              // samples per work item
 constexpr uint32_t kDecUStride = 2 * kN + 16;   // bytes between shifted copies of u
 
