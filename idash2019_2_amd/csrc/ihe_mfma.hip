@@ -381,8 +381,16 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
 // LDS holds two column tiles: while the waves multiply one, the next is recoded
 // and the one after is being loaded. Per column tile a lane issues 16 loads per
 // round and 16 stores, in a fixed order, and the workgroup meets at one barrier.
+// Waves per SIMD asked of the register allocator for the shapes that would
+// otherwise land just above an occupancy step and fit below it without
+// spilling: rotated with 6 K-blocks (261-264 VGPRs -> 247, 2 waves instead of
+// 1), unrotated with 4 (169 -> 168, 3 waves instead of 2). 1 = no request.
+template <bool ROT, int NKB>
+constexpr int mfmaw_waves() { return ROT ? (NKB == 6 ? 2 : 1) : (NKB == 4 ? 3 : 1); }
+
 template <bool ROT, int NTW, int NKB>
-__global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmaw(MfmaArgs A) {
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(mfmaw_waves<ROT, NKB>())))
+void k_cloud_mfmaw(MfmaArgs A) {
   static_assert(NKB % 2 == 0, "the four waves split NKB / 2 K-block pairs");
   constexpr uint32_t wg_per_tile = kCtWords / (NTW * 32);
   constexpr uint32_t rows = kMfmaRows * kWaves;
