@@ -12,6 +12,7 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +21,7 @@
 #include <fstream>
 #include <random>
 #include <sstream>
+#include <thread>
 
 #include "idash_hip.h"
 #include "parse_vw.h"
@@ -684,25 +686,44 @@ void write_decrypted_predictions(const DecryptedPredictions &predictions, const 
     labels.push_back(PRINT_POS_NAME ? e.second : std::to_string(e.first));
   }
   // samples are formatted in parallel, one buffer each, and written in order
+  // by a writer thread while the next block of samples is formatted
   const int64_t ns = params.NUM_SAMPLES, block = 64;
-  std::vector<string> text(block);
-  for (int64_t s0 = 0; s0 < ns; s0 += block) {
+  std::vector<string> text[2] = {std::vector<string>(block), std::vector<string>(block)};
+  std::thread writer;
+  for (int64_t s0 = 0, buf = 0; s0 < ns; s0 += block, buf ^= 1) {
     const int64_t s1 = std::min(ns, s0 + block);
+    std::vector<string> &tb = text[buf];
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t s = s0; s < s1; ++s) {
-      string &out = text[s - s0];
+      string &out = tb[s - s0];
       out.clear();
-      char line[320];
+      // "%lld,%s,%g,%g,%g\n"; to_chars' general format with precision 6 is
+      // specified to print as printf's %.6g (= %g)
+      char num[64];
+      const std::to_chars_result sr = std::to_chars(num, num + sizeof(num), (long long)s);
+      const size_t slen = (size_t)(sr.ptr - num);
       for (size_t t = 0; t < rows.size(); ++t) {
         const auto &pr = *rows[t];
-        const int n = std::snprintf(line, sizeof(line), "%lld,%s,%g,%g,%g\n", (long long)s,
-                                    labels[t].c_str(), (double)pr[0][s], (double)pr[1][s],
-                                    (double)pr[2][s]);
-        out.append(line, (size_t)std::min<int>(n, (int)sizeof(line) - 1));
+        out.append(num, slen);
+        out.push_back(',');
+        out.append(labels[t]);
+        for (int k = 0; k < 3; ++k) {
+          char v[64];
+          v[0] = ',';
+          const std::to_chars_result r = std::to_chars(v + 1, v + sizeof(v), (double)pr[k][s],
+                                                       std::chars_format::general, 6);
+          out.append(v, (size_t)(r.ptr - v));
+        }
+        out.push_back('\n');
       }
     }
-    for (int64_t s = s0; s < s1; ++s) std::fwrite(text[s - s0].data(), 1, text[s - s0].size(), f);
+    // the writer of the previous block (the other buffer) ends first
+    if (writer.joinable()) writer.join();
+    writer = std::thread([&tb, f, n = s1 - s0] {
+      for (int64_t i = 0; i < n; ++i) std::fwrite(tb[i].data(), 1, tb[i].size(), f);
+    });
   }
+  if (writer.joinable()) writer.join();
   REQUIRE_DRAMATICALLY(std::fclose(f) == 0, "result file write failed");
 }
 
