@@ -470,6 +470,7 @@ struct EncryptParams {
   double *__restrict__ out_var;
   uint32_t num_cts, num_cols, ns, R, rs;
   uint32_t seed_lo, seed_hi;
+  uint64_t rs_recip;  // ceil(2^32 / rs): j / rs = (j * rs_recip) >> 32 for every word j < N
 };
 
 // One wavefront per ciphertext c (FeatIndex):
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_encrypt(EncryptParams P
 #pragma unroll
   for (int u = 0; u < J; ++u) {
     const uint32_t j = jb + u;
-    const uint32_t r = j / P.rs, s = j - r * P.rs;
+    const uint32_t r = (uint32_t)(((uint64_t)j * P.rs_recip) >> 32), s = j - r * P.rs;
     uint32_t v = acc[u];
     if (P.geno && r < P.R && s < P.ns) {
       const uint32_t bidx = ct * P.R + r;
@@ -566,7 +567,7 @@ __global__ __launch_bounds__(256) void k_encrypt_ae(EncryptParams P) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t j = 4 * q + u;
-        const uint32_t rr = j / P.rs, sj = j - rr * P.rs;
+        const uint32_t rr = (uint32_t)(((uint64_t)j * P.rs_recip) >> 32), sj = j - rr * P.rs;
         if (rr < P.R && sj < P.ns) {
           const uint32_t bidx = ct * P.R + rr;
           if (bidx < P.num_cols) {
@@ -652,6 +653,7 @@ int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint
   P.num_cols = 3 * num_tags;
   P.R = layout->num_regions;
   P.rs = layout->region_size;
+  P.rs_recip = ((1ull << 32) + P.rs - 1) / P.rs;
   P.ns = NS;
   P.num_cts = (P.num_cols + P.R - 1) / P.R;
   P.seed_lo = (uint32_t)seed;
