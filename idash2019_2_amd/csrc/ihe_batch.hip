@@ -108,6 +108,7 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
       E.ns = L.num_samples;
       E.rs = L.region_size;
       E.R = L.num_regions;
+      E.rmagic = L.num_regions > 1 ? UINT64_MAX / L.num_regions + 1 : 0;
       E.ct_base = s.in_base / L.num_regions;
       const uint32_t in_cts = c.num_in_cts > E.ct_base ? std::min(c.num_in_cts - E.ct_base, kSegCts) : 0u;
       E.in = in_cts ? c.d_in + (size_t)E.ct_base * kCtWords : nullptr;

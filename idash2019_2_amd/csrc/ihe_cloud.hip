@@ -1381,6 +1381,7 @@ void launch_mfma_band(ihe_ctx *ctx, const ihe_band *band, const ihe_layout &L, u
     E.ns = L.num_samples;
     E.rs = L.region_size;
     E.R = L.num_regions;
+    E.rmagic = L.num_regions > 1 ? UINT64_MAX / L.num_regions + 1 : 0;
     E.ct_base = s.in_base / L.num_regions;
     const uint32_t in_cts = num_in > E.ct_base ? std::min(num_in - E.ct_base, kSegCts) : 0u;
     E.in = in_cts ? d_in + (size_t)E.ct_base * kCtWords : nullptr;

@@ -39,7 +39,9 @@ struct alignas(16) MfmaEntry {
   uint32_t in_bytes, out_bytes;
   uint32_t ns, rs, R, ct_base;  // layout; ct_base = first ciphertext of `in`
   uint32_t tile_base;           // first tile of the segment
-  uint32_t pad[13];
+  uint32_t pad0;
+  uint64_t rmagic;  // c / R = umul64hi(c, rmagic) for every 32-bit c (R > 1)
+  uint32_t pad[10];
 };
 static_assert(sizeof(MfmaEntry) == 128, "entry is two 64-byte scalar loads");
 

@@ -88,7 +88,7 @@ __device__ __forceinline__ uint32_t col_word(uint32_t c, const MfmaEntry &E) {
   if constexpr (!ROT) {
     return (c - E.ct_base) << 13;
   } else {
-    const uint32_t ct = c / E.R;
+    const uint32_t ct = (uint32_t)__umul64hi((uint64_t)c, E.rmagic);
     return ((ct - E.ct_base) << 13) | (((c - ct * E.R) * E.rs) << 2);
   }
 }
