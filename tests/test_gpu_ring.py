@@ -141,6 +141,24 @@ def test_decrypt_matches_oracle(engine, dec_kernel, ns):
     k.free()
 
 
+def test_decrypt_launches_of_different_sizes_in_a_row(engine):
+    """The MFMA kernel's work counters are cleared by the last workgroup of each
+    launch: launches with many items (a full grid), with fewer items than
+    workgroups, and many again, on one context, each equal to the oracle."""
+    rng = np.random.default_rng(11)
+    key = rng.integers(0, 2, 1024).astype(np.int32)
+    k = engine.key(key)
+    for n, ns in ((4100, 1024), (3, 64), (1, 1), (300, 700), (40, 1024)):
+        cts = rng.integers(-(1 << 31), 1 << 31, (n, 2, 1024)).astype(np.int32)
+        d_in = engine.to_device(cts)
+        d_out = engine.empty((n, ns), np.float32)
+        engine.decrypt(k, ns, d_in, n, d_out)
+        assert (d_out.numpy() == oracle.decrypt(cts, key, ns)).all(), (n, ns)
+        d_in.free()
+        d_out.free()
+    k.free()
+
+
 def test_decrypt_extreme_keys(engine, dec_kernel):
     for key in (np.zeros(1024, np.int32), np.ones(1024, np.int32)):
         rng = np.random.default_rng(int(key[0]))
