@@ -48,7 +48,8 @@ struct ihe_band {
   uint32_t *d_kcols = nullptr, *d_rowc = nullptr;
   ihe::int4v *d_adata = nullptr;
   // variances: every (big index, int32 w*w) entry of each row, in row order,
-  // interleaved per block of 64 outputs
-  uint32_t *d_var_ptr = nullptr, *d_var_col = nullptr;
+  // interleaved per block of 64 outputs; packed (d_var_base set): entries are
+  // (big index - row base) << 16 | |w|, and d_var_w2 is unused
+  uint32_t *d_var_ptr = nullptr, *d_var_col = nullptr, *d_var_base = nullptr;
   int32_t *d_var_w2 = nullptr;
 };

@@ -119,7 +119,8 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
       if (k.tiles) keyed.push_back(k);
     }
     if (c.d_out_var) {
-      vent.push_back(make_var_entry(band->d_var_ptr, band->d_var_col, band->d_var_w2, c.d_in_var,
+      vent.push_back(make_var_entry(band->d_var_ptr, band->d_var_col, band->d_var_w2,
+                                    band->d_var_base, c.d_in_var,
                                     c.d_out_var, band->num_out, L.num_regions));
       vfirst.push_back(vfirst.back() + (band->num_out + 255) / 256);
     }

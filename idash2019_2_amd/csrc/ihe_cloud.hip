@@ -471,8 +471,9 @@ struct HostBand {
   std::vector<MfmaShape> candidates;  // feasible MFMA shapes, for the autotuner
   // all layouts
   std::vector<Segment> segments;
-  std::vector<uint32_t> var_ptr, var_col, bias;
+  std::vector<uint32_t> var_ptr, var_col, var_base, bias;
   std::vector<int32_t> var_w2;
+  bool var_wide_w = false;  // some |w| > 32768: variance entries stay unpacked
   uint32_t max_window = 0, max_bidx = 0;
   bool has_cols = false;
   // grouping
@@ -514,6 +515,7 @@ int build_groups(const ihe_layout &L, uint32_t num_in, const ihe_model_csr &M, b
       // reach the output (the rotated regions carry none, eval/idash.cpp:786, :835)
       H.var_col.push_back(c);
       H.var_w2.push_back((int32_t)((uint32_t)w * (uint32_t)w));
+      if (w < -32768 || w > 32768) H.var_wide_w = true;
       H.max_bidx = std::max(H.max_bidx, c);
       H.has_cols = true;
     }
@@ -985,11 +987,24 @@ bool pick_mfma_shape(uint32_t num_out, HostBand &H, MfmaShape &shape) {
 
 // Rewrites the per-row variance entries (var_ptr / var_col / var_w2, row
 // order) into the kernel's layout: per block of 64 outputs, entry j of every
-// row side by side, rows padded with kVarPad to the block's longest row;
-// var_ptr becomes the per-block offsets in units of 64 entries.
+// row side by side, rows padded to the block's longest row; var_ptr becomes
+// the per-block offsets in units of 64 entries. When every |w| <= 32768 and
+// every row's big indices lie within 65534 of its smallest, an entry packs
+// into 32 bits, (c - base) << 16 | |w| (w*w recomputed on the device), with
+// the row bases in var_base; otherwise (c, w*w) pairs, padded with kVarPad.
 void interleave_variances(HostBand &H) {
   const uint32_t num_out = (uint32_t)H.var_ptr.size() - 1;
   const uint32_t nblk = (num_out + 63) / 64;
+  std::vector<uint32_t> base(num_out, 0);
+  bool packed = !H.var_wide_w;
+  for (uint32_t o = 0; o < num_out && packed; ++o) {
+    if (H.var_ptr[o] == H.var_ptr[o + 1]) continue;
+    uint32_t lo = UINT32_MAX, hi = 0;
+    for (uint32_t e = H.var_ptr[o]; e < H.var_ptr[o + 1]; ++e)
+      lo = std::min(lo, H.var_col[e]), hi = std::max(hi, H.var_col[e]);
+    base[o] = lo;
+    packed = hi - lo < kVarPackedPad;
+  }
   std::vector<uint32_t> off(nblk + 1, 0), col;
   std::vector<int32_t> w2;
   for (uint32_t b = 0; b < nblk; ++b) {
@@ -998,14 +1013,28 @@ void interleave_variances(HostBand &H) {
       len = std::max(len, H.var_ptr[o + 1] - H.var_ptr[o]);
     off[b + 1] = off[b] + len;
   }
-  col.assign((size_t)off[nblk] * 64, kVarPad);
-  w2.assign((size_t)off[nblk] * 64, 0);
-  for (uint32_t o = 0; o < num_out; ++o) {
-    const uint32_t b = o / 64, l = o % 64;
-    for (uint32_t e = H.var_ptr[o], j = 0; e < H.var_ptr[o + 1]; ++e, ++j) {
-      col[((size_t)off[b] + j) * 64 + l] = H.var_col[e];
-      w2[((size_t)off[b] + j) * 64 + l] = H.var_w2[e];
+  if (packed) {
+    col.assign((size_t)off[nblk] * 64, kVarPackedPad << 16);
+    for (uint32_t o = 0; o < num_out; ++o) {
+      const uint32_t b = o / 64, l = o % 64;
+      for (uint32_t e = H.var_ptr[o], j = 0; e < H.var_ptr[o + 1]; ++e, ++j) {
+        // |w| from w*w (exact below 2^31)
+        const uint32_t aw = (uint32_t)std::llround(std::sqrt((double)(uint32_t)H.var_w2[e]));
+        col[((size_t)off[b] + j) * 64 + l] = (H.var_col[e] - base[o]) << 16 | aw;
+      }
     }
+    H.var_base.swap(base);
+  } else {
+    col.assign((size_t)off[nblk] * 64, kVarPad);
+    w2.assign((size_t)off[nblk] * 64, 0);
+    for (uint32_t o = 0; o < num_out; ++o) {
+      const uint32_t b = o / 64, l = o % 64;
+      for (uint32_t e = H.var_ptr[o], j = 0; e < H.var_ptr[o + 1]; ++e, ++j) {
+        col[((size_t)off[b] + j) * 64 + l] = H.var_col[e];
+        w2[((size_t)off[b] + j) * 64 + l] = H.var_w2[e];
+      }
+    }
+    H.var_base.clear();
   }
   H.var_ptr.swap(off);
   H.var_col.swap(col);
@@ -1155,7 +1184,7 @@ int upload_band(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
   for (size_t n : {H.grp.size() * 8, H.tile_off.size(), H.cols.size(), H.oslot.size(),
                    H.obias.size(), H.wts.size(), H.tiles.size() * 8, H.stage.size(), H.meta.size(),
                    H.mtiles.size() * 8, H.kcols.size(), H.rowc.size(), H.adata.size() * 4,
-                   H.var_ptr.size(), H.var_col.size(), H.var_w2.size()})
+                   H.var_ptr.size(), H.var_col.size(), H.var_w2.size(), H.var_base.size()})
     bytes += align_up(4 * std::max<size_t>(n, 1));
   if (int rc = ihe_malloc(ctx, bytes, &b->dmem)) {
     delete b;
@@ -1187,6 +1216,8 @@ int upload_band(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
   put(&b->d_var_ptr, H.var_ptr);
   put(&b->d_var_col, H.var_col);
   put(&b->d_var_w2, H.var_w2);
+  put(&b->d_var_base, H.var_base);
+  if (H.var_base.empty()) b->d_var_base = nullptr;
   if (int rc = ihe_memcpy_h2d(ctx, b->dmem, staging.data(), bytes)) {
     (void)ihe_free(ctx, b->dmem);
     delete b;
@@ -1425,7 +1456,8 @@ int ihe_cloud_score_layout(ihe_ctx *ctx, const ihe_band *band, const ihe_layout 
   IHE_HIP(hipGetLastError());
   if (d_out_var) {
     ScopedTimer t(ctx, IHE_TIMER_CLOUD_VAR);
-    const VarEntry V = make_var_entry(band->d_var_ptr, band->d_var_col, band->d_var_w2, d_in_var,
+    const VarEntry V = make_var_entry(band->d_var_ptr, band->d_var_col, band->d_var_w2,
+                                      band->d_var_base, d_in_var,
                                       d_out_var, band->num_out, L.num_regions);
     launch_var(ctx->stream, V, nullptr, nullptr, 0, (band->num_out + 255) / 256);
     IHE_HIP(hipGetLastError());

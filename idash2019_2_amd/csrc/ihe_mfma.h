@@ -79,8 +79,9 @@ void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const Mfm
 constexpr uint32_t kVarPad = 0xFFFFFFFFu;
 struct VarEntry {
   const uint32_t *blk_off;  // [ceil(num_out / 64) + 1]
-  const uint32_t *col;      // big indices
-  const int32_t *w2;
+  const uint32_t *col;      // big indices, or packed entries when `base` is set
+  const int32_t *w2;        // int32 w*w (unpacked entries)
+  const uint32_t *base;     // packed: each output's smallest big index, else null
   const double *var_in;
   double *var_out;
   uint32_t num_out, R;
@@ -92,10 +93,12 @@ struct VarEntry {
 // 1 < R <= 2^31 (the error c * (rmagic - 2^64 / R) / 2^64 stays below 2^-32,
 // under the 1 / R gap of a non-integral quotient to the next integer).
 inline VarEntry make_var_entry(const uint32_t *blk_off, const uint32_t *col, const int32_t *w2,
-                               const double *var_in, double *var_out, uint32_t num_out,
-                               uint32_t R) {
-  return {blk_off, col, w2, var_in, var_out, num_out, R, R > 1 ? UINT64_MAX / R + 1 : 0};
+                               const uint32_t *base, const double *var_in, double *var_out,
+                               uint32_t num_out, uint32_t R) {
+  return {blk_off, col, w2, base, var_in, var_out, num_out, R, R > 1 ? UINT64_MAX / R + 1 : 0};
 }
+// A packed entry's offset field for padding.
+constexpr uint32_t kVarPackedPad = 0xFFFFu;
 void launch_var(hipStream_t stream, const VarEntry &single, const VarEntry *entries,
                 const uint32_t *first, uint32_t num_entries, uint32_t num_blocks);
 

@@ -550,9 +550,21 @@ __global__ __launch_bounds__(256) void k_cloud_var(VarArgs A) {
   double v = 0.0;
   const uint32_t e1 = E.blk_off[blk + 1];
   uint32_t e = E.blk_off[blk];
-  auto take = [&](uint32_t c, bool &keep, double &x) {
+  // packed entries: (c - base) << 16 | |w|; padding has offset kVarPackedPad
+  const bool packed = E.base != nullptr;
+  const uint32_t cb = packed && o < E.num_out ? E.base[o] : 0;
+  auto take = [&](uint32_t raw, int32_t w2in, bool &keep, double &x, int32_t &w2) {
+    uint32_t c = raw;
+    bool pad = c == kVarPad;
+    w2 = w2in;
+    if (packed) {
+      pad = (raw >> 16) == kVarPackedPad;
+      c = cb + (raw >> 16);
+      const uint32_t aw = raw & 0xFFFFu;
+      w2 = (int32_t)(aw * aw);
+    }
     const uint32_t ct = div_r(c, E.R, E.rmagic);
-    keep = c != kVarPad && c == ct * E.R;
+    keep = !pad && c == ct * E.R;
     x = keep ? E.var_in[ct] : 0.0;
   };
   for (; e + kU <= e1; e += kU) {
@@ -561,22 +573,22 @@ __global__ __launch_bounds__(256) void k_cloud_var(VarArgs A) {
 #pragma unroll
     for (int j = 0; j < kU; ++j) {
       c[j] = E.col[(size_t)(e + j) * 64 + lane];
-      w2[j] = E.w2[(size_t)(e + j) * 64 + lane];
+      w2[j] = packed ? 0 : E.w2[(size_t)(e + j) * 64 + lane];
     }
     bool keep[kU];
     double x[kU];
 #pragma unroll
-    for (int j = 0; j < kU; ++j) take(c[j], keep[j], x[j]);
+    for (int j = 0; j < kU; ++j) take(c[j], w2[j], keep[j], x[j], w2[j]);
 #pragma unroll
     for (int j = 0; j < kU; ++j)
       if (keep[j]) v = __dadd_rn(v, __dmul_rn((double)w2[j], x[j]));
   }
   for (; e < e1; ++e) {
     const uint32_t c = E.col[(size_t)e * 64 + lane];
-    const int32_t w2 = E.w2[(size_t)e * 64 + lane];
+    int32_t w2 = packed ? 0 : E.w2[(size_t)e * 64 + lane];
     bool keep;
     double x;
-    take(c, keep, x);
+    take(c, w2, keep, x, w2);
     if (keep) v = __dadd_rn(v, __dmul_rn((double)w2, x));
   }
   if (o < E.num_out) E.var_out[o] = v;

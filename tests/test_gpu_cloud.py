@@ -129,6 +129,36 @@ def test_random_sparse_model_matches_oracle(engine, ns, variant, wmax):
     assert (ov == rv).all()
 
 
+@pytest.mark.parametrize("span", [300, 70000])
+def test_variance_entry_layouts_match_oracle(engine, span):
+    """Output variances (tLweAddMulTo's w*w * var, region-0 columns only) with
+    the band's packed entries (a row's big indices within 65534 of each other)
+    and, for rows spanning 70000 big indices, its unpacked ones; NS = 16, so
+    R = 64 and only every 64th column reaches the variance."""
+    ns, R = 16, 64
+    rng = np.random.default_rng(span)
+    num_in = (span + R - 1) // R
+    ct = random_cts(rng, num_in)
+    var = rng.integers(1, 1 << 20, num_in) * 2.0 ** -60
+    rows, ib, w = [0], [], []
+    for o in range(150):
+        cols = rng.choice(span, size=int(rng.integers(1, 30)), replace=False)
+        if o % 3 == 0:
+            cols = np.concatenate([cols[:1] * 0, cols, [span - 1]])    # the full span
+            cols = np.unique(cols)
+        cols = cols[rng.permutation(cols.size)]                      # row order, unsorted
+        ib += [int(x) for x in cols]
+        w += [int(rng.integers(-32768, 32640)) for _ in cols]
+        ib.append(_abi.IHE_CONSTANT_BIDX)
+        w.append(int(rng.integers(-2048, 2048)))
+        rows.append(len(ib))
+    rp, ib, w = np.array(rows, np.uint32), np.array(ib, np.uint32), np.array(w, np.int32)
+    out, ov, _ = run_cloud(engine, ct, var, rp, ib, w, ns)
+    ref, rv = oracle.cloud(ct, var, rp, ib, w, ns, R, 1024 // R)
+    assert (out == ref).all()
+    assert (ov == rv).all()
+
+
 def test_wide_window_matches_oracle(engine, kernel):
     """50 neighbours (150 columns, several register chunks per group)."""
     c = synth.generate(120, 200, 1024, 50, seed=50)
