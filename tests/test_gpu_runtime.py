@@ -1,6 +1,9 @@
 """Runtime utilities on the GPU: the device checksum (ihe_checksum) against its
-closed form computed on the host, and the HBM streaming probe
-(ihe_stream_probe): timed, bounded to the bytes it is given, argument errors."""
+closed form computed on the host, the HBM streaming probe (ihe_stream_probe):
+timed, bounded to the bytes it is given, argument errors; and launches on a
+caller's stream (ihe_ctx_set_stream / ihe_ctx_get_stream)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -57,3 +60,48 @@ def test_stream_probe_is_timed_and_stays_in_bounds(engine):
     for args in ((in_b + 1, out_b), (in_b, 1000), (in_b, out_b + 512), (in_b, 1 << 31)):
         with pytest.raises((IheError, ValueError)):
             engine.stream_probe(d_in, args[0], d_out, args[1])
+
+
+def test_user_stream_switch(engine):
+    """ihe_ctx_set_stream: launches go to a caller's stream (a decrypt and a
+    cloud step, equal to the oracle), ihe_ctx_get_stream reports it, and NULL
+    returns the context to its own stream (the switch waits for the old one)."""
+    import oracle
+    from idash2019_2_amd import synth
+    hip = ctypes.CDLL("libamdhip64.so.7")         # the runtime the library links
+    hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    s = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(s)) == 0
+    L = _abi.lib()
+    try:
+        _abi.check(L.ihe_ctx_set_stream(engine.ctx, s))
+        got = ctypes.c_void_p()
+        _abi.check(L.ihe_ctx_get_stream(engine.ctx, ctypes.byref(got)))
+        assert got.value == s.value
+        c = synth.generate(30, 60, 128, 5, seed=8)
+        rng = np.random.default_rng(8)
+        ct = rng.integers(-(1 << 31), 1 << 31, (c.num_in_cts, 2, 1024)).astype(np.int32)
+        var = np.full(c.num_in_cts, 2.0 ** -50)
+        rp, ib, w = c.model_csr()
+        for phase in ("user stream", "own stream"):
+            band = engine.band(128, c.num_in_cts, rp, ib, w)
+            d_in, d_var = engine.to_device(ct), engine.to_device(var)
+            d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
+            d_ov = engine.empty((c.num_outputs,), np.float64)
+            engine.cloud(band, d_in, d_var, d_out, d_ov)
+            key = engine.key(c.key)
+            d_ph = engine.empty((c.num_outputs, 128), np.float32)
+            engine.decrypt(key, 128, d_out, c.num_outputs, d_ph)
+            engine.sync()
+            ref, rv = oracle.cloud(ct, var, rp, ib, w, *c.layout)
+            assert (d_out.numpy() == ref).all() and (d_ov.numpy() == rv).all(), phase
+            assert (d_ph.numpy() == oracle.decrypt(ref, c.key, 128)).all(), phase
+            key.free()
+            band.free()
+            _abi.check(L.ihe_ctx_set_stream(engine.ctx, None))
+        _abi.check(L.ihe_ctx_get_stream(engine.ctx, ctypes.byref(got)))
+        assert got.value != s.value
+    finally:
+        L.ihe_ctx_set_stream(engine.ctx, None)
+        hip.hipStreamDestroy(s)
