@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -173,3 +174,13 @@ def test_band_plan_empty_and_bias_only_outputs():
     info = band_plan(64, 0, [0, 1, 1], [_abi.IHE_CONSTANT_BIDX], [7])
     # one group without columns
     assert info["num_outputs"] == 2 and info["num_groups"] == 1 and info["max_window"] == 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message (before any
+    device work), as multi-GPU runs are not part of this build."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr and r.stdout == ""
