@@ -20,7 +20,9 @@ from idash2019_2_amd import formats, synth
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HOST = os.path.join(ROOT, "idash2019_2_amd", "host")
+# IHE_HOST_DIR runs the same cases on another build of the executables
+# (make -C idash2019_2_amd/host asan: AddressSanitizer + UBSan)
+HOST = os.path.join(ROOT, os.environ.get("IHE_HOST_DIR", os.path.join("idash2019_2_amd", "host")))
 ONE, NAN = 1 << 18, (131072, 87381, 43690)
 
 

@@ -11,7 +11,9 @@ import pytest
 from idash2019_2_amd import formats, synth
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HOST = os.path.join(ROOT, "idash2019_2_amd", "host")
+# IHE_HOST_DIR runs the same cases on another build of the executables
+# (make -C idash2019_2_amd/host asan: AddressSanitizer + UBSan)
+HOST = os.path.join(ROOT, os.environ.get("IHE_HOST_DIR", os.path.join("idash2019_2_amd", "host")))
 
 
 def run(exe, *args, cwd, env=None):
