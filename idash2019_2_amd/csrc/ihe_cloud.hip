@@ -48,9 +48,10 @@ constexpr int kWideChunkCols = 8;            // 8 columns per chunk (32-bit weig
 constexpr int kRsrcWord3 = 0x00020000;       // gfx9 raw buffer descriptor, dword 3
 
 #define IHE_CONSTANT __attribute__((address_space(4)))
-// Cache policy of the output stores (2 = nt, streaming).
+// Cache policy of the output stores: nt (2, streaming) with sc1 (16, device
+// scope), as for the MFMA kernels.
 #ifndef IHE_STORE_POLICY
-#define IHE_STORE_POLICY 2
+#define IHE_STORE_POLICY 18
 #endif
 
 // Uniform metadata is read through the constant address space so the loads

@@ -31,7 +31,10 @@ typedef uint32_t u32x16v __attribute__((ext_vector_type(16)));
 constexpr int kRsrcWord3 = 0x00020000;          // gfx9 raw buffer descriptor, dword 3
 constexpr uint32_t kDropOffset = 0x80000000u;   // >= every segment's byte size: stores dropped
 constexpr uint32_t kWaves = 4;                  // waves per workgroup
-constexpr int kStorePolicy = 2;                 // nt: streaming output stores
+// Output stores: nt (2, streaming) with sc1 (16, device scope: written through
+// the XCD's L2). Outputs are never re-read by the step; of the documented
+// cache-policy bits this pair streams the step's volumes fastest.
+constexpr int kStorePolicy = 18;
 
 template <typename T>
 __device__ __forceinline__ const IHE_CONSTANT T *cptr(const T *p) {

@@ -302,7 +302,9 @@ __global__ __launch_bounds__(256) void k_checksum(const uint32_t *__restrict__ p
 }
 
 // One wave per 64 KiB of output: 64 stores of 1 KiB, with the wave's share
-// of the input read in 1 KiB loads spread evenly between them.
+// of the input read in 1 KiB loads spread evenly between them. Stores carry
+// nt | sc1 (18), the fastest documented cache policy for this mix (the
+// cloud kernels' too); loads the default, which beats nt here.
 __global__ __launch_bounds__(256) void k_stream_probe(const int *in, uint32_t in_bytes, int *out,
                                                       uint32_t out_bytes, uint32_t num_waves) {
   typedef int int4v __attribute__((ext_vector_type(4)));
@@ -320,7 +322,7 @@ __global__ __launch_bounds__(256) void k_stream_probe(const int *in, uint32_t in
   for (uint32_t j = 0; j < 64; ++j) {
     const uint32_t until = r0 + (uint32_t)((uint64_t)(r1 - r0) * (j + 1) / 64);
     for (; r < until; ++r) acc += __builtin_amdgcn_raw_buffer_load_b128(ri, r * 1024u + lane * 16u, 0, 0);
-    __builtin_amdgcn_raw_buffer_store_b128(acc, ro, (wave * 64u + j) * 1024u + lane * 16u, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(acc, ro, (wave * 64u + j) * 1024u + lane * 16u, 0, 18);
   }
 }
 
