@@ -480,7 +480,7 @@ def main(argv=None):
         eng.timing(True)
         eng.timing_reset()
         d_dec = eng.empty((c.num_outputs, ns), np.float32)
-        for _ in range(3):
+        for _ in range(10):
             eng.encrypt(key, ns, d_geno, 0x5EED, d_ct, d_var)
             eng.decrypt(key, ns, d_out, c.num_outputs, d_dec)
         enc_ms, enc_n = eng.timing_read(_abi.TIMER_ENCRYPT)

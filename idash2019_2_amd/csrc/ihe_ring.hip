@@ -307,7 +307,7 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
       return ((((((uint32_t)acc[t][3][v] << 8) + (uint32_t)acc[t][2][v]) << 8) +
                (uint32_t)acc[t][1][v]) << 8) + (uint32_t)acc[t][0][v];
     };
-    if (whole(ct0, jw)) {
+    if (whole(ct0, jw)) {  // streaming (nt) stores: outputs are not re-read here
       char *out_base = ENC ? (char *)(P.bout + (size_t)ct0 * kCtWords)
                            : (char *)(P.out + (size_t)ct0 * P.ns);
       const uint32_t out_lane = (4 * h * P.ns + jw + c) * 4;
@@ -317,10 +317,12 @@ void k_ring_mfma(DecryptParams P, KeyBits key) {
         for (int v = 0; v < 16; ++v) {
           const uint32_t as = as_of(t, v), b = t == 0 ? bw[v] : bw1[v];
           if constexpr (ENC)
-            *(int32_t *)(out_base + in_off(jw, v) + 128 * t) = (int32_t)(b + as);
+            __builtin_nontemporal_store((int32_t)(b + as),
+                                        (int32_t *)(out_base + in_off(jw, v) + 128 * t));
           else
-            *(float *)(out_base + out_lane + (8 * (v / 4) + (v % 4)) * P.ns * 4 + 128 * t) =
-                (float)(int32_t)(b - as) * 0x1p-32f;
+            __builtin_nontemporal_store(
+                (float)(int32_t)(b - as) * 0x1p-32f,
+                (float *)(out_base + out_lane + (8 * (v / 4) + (v % 4)) * P.ns * 4 + 128 * t));
         }
       }
       return;
