@@ -442,7 +442,8 @@ def main(argv=None):
         p_out.free()
         probe_ms /= probe_n
         ceiling = {"probe": "contiguous 16-byte reads and writes interleaved per wave, the "
-                            "volumes one cloud step reads and writes",
+                            "volumes one cloud step reads and writes; streaming (nt) "
+                            "device-scope (sc1) stores, as the cloud kernel's",
                    "read_bytes": in_b, "write_bytes": out_b, "ms": round(probe_ms, 4),
                    "GB_per_s": round((in_b + out_b) / probe_ms / 1e6, 1),
                    "cloud_kernel_frac": round(probe_ms / (avg_kernel_s * 1e3), 4)}
