@@ -148,3 +148,18 @@ def test_model_cache_is_written_validated_and_refreshed(tmp_path):
     run("cloud", "model", cwd=tmp_path)
     assert read_model_cache(cache)[(pos, 1)][1][0xFFFFFFFF] == 4321
     assert p["NUM_OUTPUT_FEATURES"] == 3 * c.num_targets
+
+
+def test_sanitizer_build_recipe():
+    """make asan (a dry run): the host library and executables with AddressSanitizer
+    and UBSan, and the engine library with the same on its host code only."""
+    src = os.path.join(ROOT, "idash2019_2_amd", "host")
+    r = subprocess.run(["make", "-n", "-B", "-C", src, "asan"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    hip = [l for l in lines if "hipcc" in l and " -c " in l]
+    assert len(hip) == 6 and all("-Xarch_host -fsanitize=address" in l for l in hip)
+    assert not any(" -fsanitize=" in l.replace("-Xarch_host -fsanitize=", "") for l in hip)
+    for exe in ("keygen", "encrypt", "cloud", "decrypt"):
+        assert any(f"-o asan/{exe} " in l and "-fsanitize=address,undefined" in l for l in lines), exe
