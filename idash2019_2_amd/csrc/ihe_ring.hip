@@ -554,6 +554,22 @@ __global__ __launch_bounds__(256) void k_encrypt_ae(EncryptParams P) {
   const uint32_t q = threadIdx.x;
   for (uint32_t ct = blockIdx.x; ct < P.num_cts; ct += gridDim.x) {
     int32_t *a = P.out + (size_t)ct * kCtWords;
+    // The genotype bytes of this thread's four b words (setScore) are loaded
+    // first, so that their latency runs under the Philox rounds below.
+    int32_t g[4] = {0, 0, 0, 0};
+    uint32_t val[4] = {0, 0, 0, 0};
+    bool has[4] = {false, false, false, false};
+    if (P.geno) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t j = 4 * q + u;
+        const uint32_t rr = (uint32_t)(((uint64_t)j * P.rs_recip) >> 32), sj = j - rr * P.rs;
+        const uint32_t bidx = ct * P.R + rr, tag = bidx / 3;
+        has[u] = rr < P.R && sj < P.ns && bidx < P.num_cols;
+        val[u] = bidx - 3 * tag;
+        if (has[u]) g[u] = P.geno[(size_t)tag * P.ns + sj];
+      }
+    }
     uint32_t r[4];
     philox(q, ct, 0, 0, P.seed_lo, P.seed_hi, r);
     *(u32x4v *)(a + 4 * q) = u32x4v{r[0], r[1], r[2], r[3]};
@@ -565,23 +581,13 @@ __global__ __launch_bounds__(256) void k_encrypt_ae(EncryptParams P) {
       b[2 * pr + 1] =
           (uint32_t)noise_sample_lut((uint64_t)r[2] | ((uint64_t)r[3] << 32), s_table, s_lut);
     }
-    if (P.geno) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t j = 4 * q + u;
-        const uint32_t rr = (uint32_t)(((uint64_t)j * P.rs_recip) >> 32), sj = j - rr * P.rs;
-        if (rr < P.R && sj < P.ns) {
-          const uint32_t bidx = ct * P.R + rr;
-          if (bidx < P.num_cols) {
-            const uint32_t tag = bidx / 3, val = bidx - 3 * tag;
-            const int8_t g = P.geno[(size_t)tag * P.ns + sj];
-            if (g == 0 || g == 1 || g == 2) {
-              if ((uint32_t)g == val) b[u] += kOne;
-            } else {
-              b[u] += val == 0 ? kNan[0] : (val == 1 ? kNan[1] : kNan[2]);
-            }
-          }
-        }
+    for (int u = 0; u < 4; ++u) {
+      if (!has[u]) continue;
+      if (g[u] == 0 || g[u] == 1 || g[u] == 2) {
+        if ((uint32_t)g[u] == val[u]) b[u] += kOne;
+      } else {
+        b[u] += val[u] == 0 ? kNan[0] : (val[u] == 1 ? kNan[1] : kNan[2]);
       }
     }
     *(u32x4v *)(a + kN + 4 * q) = u32x4v{b[0], b[1], b[2], b[3]};
