@@ -1,5 +1,7 @@
 // CPU baseline for the cloud step. TEST/BENCH INFRASTRUCTURE ONLY: bench.py
-// times it as cpu_baseline and tests use it as a second CPU checker.
+// times it as cpu_baseline (and fails when the GPU output differs from it);
+// tests/test_oracle.py holds it bit-exact to orc_cloud_reference at R = 1, 8
+// and 64.
 //
 // It keeps the data structures and threading of cloud_compute_score
 // (eval/idash.cpp:763-848): the model is an
@@ -35,7 +37,9 @@ double wall() {
 }
 
 inline void mul_by_xai(int32_t *out, int32_t a, const int32_t *in) {
-  // torusPolynomialMulByXai for a in [N, 2N), the only range cloud uses.
+  // torusPolynomialMulByXai for a in [N, 2N): the cloud step rotates region
+  // r in [1, R) by a = 2N - r*RS (eval/idash.cpp:832), and r*RS < R*RS <= N
+  // puts every such a in (N, 2N).
   const int32_t aa = a - kN;
   for (int32_t i = 0; i < aa; ++i) out[i] = in[i - aa + kN];
   for (int32_t i = aa; i < kN; ++i) out[i] = (int32_t)(0u - (uint32_t)in[i - aa]);

@@ -168,12 +168,17 @@ void orc_phase(int32_t N, const int32_t *ct, const int32_t *key, int32_t *phase)
 /* decrypt_predictions inner loop (eval/idash.cpp:711-719). */
 void orc_decrypt(int32_t N, uint32_t NS, const int32_t *cts, uint32_t num_cts,
                  const int32_t *key, float *out) {
-    int32_t *phase = (int32_t *)malloc(sizeof(int32_t) * N);
-    for (uint32_t c = 0; c < num_cts; ++c) {
-        orc_phase(N, cts + (size_t)c * 2 * N, key, phase);
-        for (uint32_t j = 0; j < NS; ++j) out[(size_t)c * NS + j] = (float)orc_t32tod(phase[j]);
+    /* ciphertexts are independent (the reference's omp for, eval/idash.cpp:701-705) */
+#pragma omp parallel
+    {
+        int32_t *phase = (int32_t *)malloc(sizeof(int32_t) * N);
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t c = 0; c < (int64_t)num_cts; ++c) {
+            orc_phase(N, cts + (size_t)c * 2 * N, key, phase);
+            for (uint32_t j = 0; j < NS; ++j) out[(size_t)c * NS + j] = (float)orc_t32tod(phase[j]);
+        }
+        free(phase);
     }
-    free(phase);
 }
 
 /* Philox4x32-10: ten rounds of (hi,lo) = mulhilo(M, x) with Weyl key bumps. */
@@ -218,53 +223,75 @@ static int32_t noise_sample(uint64_t u, const uint64_t *table) {
     return (u & 1) ? -lo : lo;
 }
 
-void orc_encrypt(int32_t N, uint32_t NS, uint32_t R, uint32_t RS,
-                 uint32_t num_tags, const int8_t *geno, const int32_t *key,
-                 uint64_t seed, int32_t *out, double *var_out) {
+/* One ciphertext c of encrypt_data_ph1 + ph2: a from Philox stream 0, b =
+ * a*s + e (e from Philox stream 1 through the noise table), then the one-hot
+ * messages of the big indices c*R .. c*R + R - 1 at slots r*RS + j. */
+static void encrypt_one(int32_t N, uint32_t NS, uint32_t R, uint32_t RS, uint32_t num_tags,
+                        const int8_t *geno, const int32_t *key, const uint32_t skey[2],
+                        const uint64_t *table, const int32_t C4[4], uint32_t c, int32_t *as,
+                        int32_t *a, double *var) {
+    const uint32_t ncols = 3 * num_tags;
+    int32_t *b = a + N;
+    for (int32_t w = 0; w < N; w += 4) {
+        const uint32_t ctr[4] = {(uint32_t)(w / 4), c, 0, 0};
+        uint32_t r4[4];
+        orc_philox4x32_10(ctr, skey, r4);
+        for (int q = 0; q < 4 && w + q < N; ++q) a[w + q] = (int32_t)r4[q];
+    }
+    orc_negacyclic_mul(as, a, key, N);
+    for (int32_t j = 0; j < N; j += 2) {
+        const uint32_t ctr[4] = {(uint32_t)(j / 2), c, 1, 0};
+        uint32_t r4[4];
+        orc_philox4x32_10(ctr, skey, r4);
+        const uint64_t u0 = (uint64_t)r4[0] | ((uint64_t)r4[1] << 32);
+        const uint64_t u1 = (uint64_t)r4[2] | ((uint64_t)r4[3] << 32);
+        b[j] = (int32_t)((uint32_t)as[j] + (uint32_t)noise_sample(u0, table));
+        if (j + 1 < N) b[j + 1] = (int32_t)((uint32_t)as[j + 1] + (uint32_t)noise_sample(u1, table));
+    }
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t bidx = c * R + r;
+        if (bidx >= ncols) break;
+        const uint32_t tag = bidx / 3, v = bidx % 3;
+        for (uint32_t j = 0; j < NS; ++j) {
+            const int8_t g = geno[(size_t)tag * NS + j];
+            uint32_t add = 0;
+            if (g == 0 || g == 1 || g == 2) {
+                if ((uint32_t)g == v) add = (uint32_t)C4[0];
+            } else {
+                add = (uint32_t)C4[1 + v];
+            }
+            b[r * RS + j] = (int32_t)((uint32_t)b[r * RS + j] + add);
+        }
+    }
+    *var = ldexp(1.0, -50);
+}
+
+void orc_encrypt_cts(int32_t N, uint32_t NS, uint32_t R, uint32_t RS,
+                     uint32_t num_tags, const int8_t *geno, const int32_t *key,
+                     uint64_t seed, const uint32_t *cts, uint32_t count,
+                     int32_t *out, double *var_out) {
     int32_t C4[4];
     orc_encoding_constants(C4);
     uint64_t table[ORC_NOISE_TABLE_LEN];
     orc_noise_table(table);
-    const uint32_t ncols = 3 * num_tags;
-    const uint32_t num_cts = (ncols + R - 1) / R;
     const uint32_t skey[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-    int32_t *as = (int32_t *)malloc(sizeof(int32_t) * N);
-    for (uint32_t c = 0; c < num_cts; ++c) {
-        int32_t *a = out + (size_t)c * 2 * N, *b = a + N;
-        for (int32_t w = 0; w < N; w += 4) {
-            const uint32_t ctr[4] = {(uint32_t)(w / 4), c, 0, 0};
-            uint32_t r4[4];
-            orc_philox4x32_10(ctr, skey, r4);
-            for (int q = 0; q < 4 && w + q < N; ++q) a[w + q] = (int32_t)r4[q];
-        }
-        orc_negacyclic_mul(as, a, key, N);
-        for (int32_t j = 0; j < N; j += 2) {
-            const uint32_t ctr[4] = {(uint32_t)(j / 2), c, 1, 0};
-            uint32_t r4[4];
-            orc_philox4x32_10(ctr, skey, r4);
-            const uint64_t u0 = (uint64_t)r4[0] | ((uint64_t)r4[1] << 32);
-            const uint64_t u1 = (uint64_t)r4[2] | ((uint64_t)r4[3] << 32);
-            b[j] = (int32_t)((uint32_t)as[j] + (uint32_t)noise_sample(u0, table));
-            if (j + 1 < N) b[j + 1] = (int32_t)((uint32_t)as[j + 1] + (uint32_t)noise_sample(u1, table));
-        }
-        for (uint32_t r = 0; r < R; ++r) {
-            const uint32_t bidx = c * R + r;
-            if (bidx >= ncols) break;
-            const uint32_t tag = bidx / 3, v = bidx % 3;
-            for (uint32_t j = 0; j < NS; ++j) {
-                const int8_t g = geno[(size_t)tag * NS + j];
-                uint32_t add = 0;
-                if (g == 0 || g == 1 || g == 2) {
-                    if ((uint32_t)g == v) add = (uint32_t)C4[0];
-                } else {
-                    add = (uint32_t)C4[1 + v];
-                }
-                b[r * RS + j] = (int32_t)((uint32_t)b[r * RS + j] + add);
-            }
-        }
-        var_out[c] = ldexp(1.0, -50);
+    /* ciphertexts are independent (the reference's omp for, eval/idash.cpp:623) */
+#pragma omp parallel
+    {
+        int32_t *as = (int32_t *)malloc(sizeof(int32_t) * N);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t i = 0; i < (int64_t)count; ++i)
+            encrypt_one(N, NS, R, RS, num_tags, geno, key, skey, table, C4,
+                        cts ? cts[i] : (uint32_t)i, as, out + (size_t)i * 2 * N, var_out + i);
+        free(as);
     }
-    free(as);
+}
+
+void orc_encrypt(int32_t N, uint32_t NS, uint32_t R, uint32_t RS,
+                 uint32_t num_tags, const int8_t *geno, const int32_t *key,
+                 uint64_t seed, int32_t *out, double *var_out) {
+    const uint32_t num_cts = (3 * num_tags + R - 1) / R;
+    orc_encrypt_cts(N, NS, R, RS, num_tags, geno, key, seed, NULL, num_cts, out, var_out);
 }
 
 /* compute_score (eval/idash.cpp:888-927) with the one-hot of

@@ -117,6 +117,12 @@ void orc_noise_table(uint64_t table[ORC_NOISE_TABLE_LEN]);
 void orc_encrypt(int32_t N, uint32_t NS, uint32_t R, uint32_t RS,
                  uint32_t num_tags, const int8_t *geno, const int32_t *key,
                  uint64_t seed, int32_t *out, double *var_out);
+/* The same for the listed ciphertexts only: out[i] and var_out[i] receive
+ * ciphertext cts[i] of orc_encrypt's output (cts NULL: ciphertext i). */
+void orc_encrypt_cts(int32_t N, uint32_t NS, uint32_t R, uint32_t RS,
+                     uint32_t num_tags, const int8_t *geno, const int32_t *key,
+                     uint64_t seed, const uint32_t *cts, uint32_t count,
+                     int32_t *out, double *var_out);
 
 /* Plaintext scorer compute_score (eval/idash.cpp:888-927), one output slot:
  * score[j] = sum over entries of w * onehot(bidx)[j] (bias: w * ONE_IN_D). */

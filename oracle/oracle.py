@@ -54,6 +54,7 @@ def _declare(L):
     L.orc_philox4x32_10.argtypes = [vp, vp, vp]
     L.orc_noise_table.argtypes = [vp]
     L.orc_encrypt.argtypes = [i32, u32, u32, u32, u32, vp, vp, u64, vp, vp]
+    L.orc_encrypt_cts.argtypes = [i32, u32, u32, u32, u32, vp, vp, u64, vp, u32, vp, vp]
     L.orc_plaintext_score.argtypes = [u32, u32, vp, u32, vp, vp, vp]
     L.orc_cpu_baseline_cloud.argtypes = [ctypes.c_int, u32, u32, u32, vp, vp, u32, u32, vp, vp,
                                          vp, vp, vp]
@@ -164,6 +165,18 @@ def encrypt(geno, key, ns, R, RS, seed, n=N):
     out = np.empty((C, 2, n), np.int32)
     var = np.empty(C, np.float64)
     lib().orc_encrypt(n, ns, R, RS, tags, _p(geno), _p(key), seed, _p(out), _p(var))
+    return out, var
+
+
+def encrypt_cts(geno, key, ns, R, RS, seed, cts, n=N):
+    """Ciphertexts cts[i] of encrypt()'s output only: ([len(cts), 2, n], [len(cts)])."""
+    geno = np.ascontiguousarray(geno, np.int8)
+    key = np.ascontiguousarray(key, np.int32)
+    cts = np.ascontiguousarray(cts, np.uint32)
+    out = np.empty((cts.size, 2, n), np.int32)
+    var = np.empty(cts.size, np.float64)
+    lib().orc_encrypt_cts(n, ns, R, RS, geno.shape[0], _p(geno), _p(key), seed, _p(cts), cts.size,
+                          _p(out), _p(var))
     return out, var
 
 

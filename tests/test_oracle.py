@@ -335,3 +335,40 @@ def test_encrypt_cloud_decrypt_round_trip_matches_plaintext_score():
         score = oracle.plaintext_score(c.geno, in_bidx[s:e], weight[s:e], ns)
         bound = (np.abs(weight[s:e]).sum() * 1280) / 2.0 ** 32 + 1e-6
         assert np.abs(dec[o] - score).max() <= bound
+
+
+def test_encrypt_subset_equals_full_encrypt():
+    """orc_encrypt_cts (the ciphertexts the sampled rows of a full-size check
+    read) gives the listed rows of orc_encrypt, in list order."""
+    rng = np.random.default_rng(3)
+    for ns in (1024, 100, 16):
+        R, RS = layout(1024, ns)
+        tags = 30
+        geno = rng.integers(-1, 3, (tags, ns)).astype(np.int8)
+        key = rng.integers(0, 2, 1024).astype(np.int32)
+        cts, var = oracle.encrypt(geno, key, ns, R, RS, seed=77)
+        pick = rng.permutation(cts.shape[0])[:max(1, cts.shape[0] // 2)]
+        sub, sv = oracle.encrypt_cts(geno, key, ns, R, RS, 77, pick)
+        assert (sub == cts[pick]).all() and (sv == var[pick]).all()
+
+
+# ---------------------------------------------------------------- CPU baseline
+
+@pytest.mark.parametrize("ns", [1024, 128, 16])
+def test_cpu_baseline_equals_reference(ns):
+    """The timed baseline (hash-map model, OpenMP over outputs, its own
+    rotation) against the literal restatement, at R = 1, 8 and 64, with
+    random ciphertexts, variances and unordered rows."""
+    from idash2019_2_amd import synth
+    c = synth.generate(40, 90, ns, 5, seed=ns)
+    _, R, RS = c.layout
+    rng = np.random.default_rng(ns)
+    ct = rng.integers(-(1 << 31), 1 << 31, (c.num_in_cts, 2, 1024), dtype=np.int64).astype(np.int32)
+    var = rng.integers(1, 1 << 20, c.num_in_cts) * 2.0 ** -60
+    rp, ib, w = c.model_csr()
+    ref, rv = oracle.cloud(ct, var, rp, ib, w, ns, R, RS)
+    for threads in (1, 3):
+        out, ov, secs = oracle.cpu_baseline_cloud(ct, var, rp, ib, w, ns, R, RS, threads)
+        assert secs >= 0
+        assert (out == ref).all()
+        assert (ov == rv).all()
