@@ -9,6 +9,7 @@ reference's formats (eval/idash.cpp:128-246 params, :476-502 keys,
                double alpha_max, N int32 key coefficients (this build's key stream)
   *.bin data   uint64 count, then per record uint32 index + one 8204-byte sample
                stream: int32 framing word, a[N], b[N] (int32), double variance
+Also a writer of ciphertext record files (test inputs in the reference format).
 """
 from __future__ import annotations
 
@@ -78,3 +79,27 @@ def read_records(path: str, slots: int | None = None):
     ct[idx] = words
     v[idx] = var
     return idx, ct, v
+
+
+def read_stream_words(path: str) -> np.ndarray:
+    """The framing word that opens each record's sample stream, in file order."""
+    with open(path, "rb") as f:
+        (count,) = struct.unpack("<Q", f.read(8))
+        raw = np.frombuffer(f.read(count * RECORD_BYTES), np.uint8).reshape(count, RECORD_BYTES)
+    return raw[:, 4:8].copy().view(np.int32).ravel()
+
+
+def write_records(path: str, idx, ct, var, stream_word: int = 0) -> None:
+    """Writes records (index idx[i], sample ct[idx[i]], variance var[idx[i]]) in
+    the order of idx, each sample stream opened with stream_word."""
+    idx = np.asarray(idx, np.uint32)
+    ct = np.ascontiguousarray(ct, np.int32)
+    var = np.ascontiguousarray(var, np.float64)
+    raw = np.empty((idx.size, RECORD_BYTES), np.uint8)
+    raw[:, 0:4] = idx.view(np.uint8).reshape(-1, 4)
+    raw[:, 4:8] = np.full(idx.size, stream_word, np.int32).view(np.uint8).reshape(-1, 4)
+    raw[:, 8:8 + 8 * N] = ct[idx].reshape(idx.size, -1).view(np.uint8)
+    raw[:, 8 + 8 * N:] = var[idx].view(np.uint8).reshape(-1, 8)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<Q", idx.size))
+        f.write(raw.tobytes())

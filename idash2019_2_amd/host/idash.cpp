@@ -64,7 +64,6 @@ constexpr size_t kCtWords = 2 * IdashParams::N;
 constexpr size_t kCtBytes = kCtWords * sizeof(int32_t);
 // One sample in a file: 4-byte framing word, a, b, variance (8204 bytes).
 constexpr size_t kSampleStreamBytes = 4 + kCtBytes + sizeof(double);
-constexpr int32_t kSampleStreamTag = 0;
 
 [[noreturn]] void die_engine(const char *what) {
   std::cout << "ERROR: " << what << ": " << ihe_last_error() << std::endl;
@@ -339,13 +338,21 @@ struct HrFile {
   std::vector<std::pair<FeatBigIndex, int32_t>> entries;  // in read() order
 };
 
+// Order-independent signature of the tag index: every (position, class, big
+// index) triple and the feature counts.
 uint64_t index_signature(const IdashParams &p) {
-  uint64_t h = 0x9E3779B97F4A7C15ull ^ p.NUM_INPUT_POSITIONS;
-  for (const auto &e : p.in_features_index) {  // order-independent mix
-    uint64_t x = e.first * 0xBF58476D1CE4E5B9ull ^ ((uint64_t)e.second[0] << 32 | e.second[2]);
-    x ^= x >> 31;
-    h += x * 0x94D049BB133111EBull;
-  }
+  auto mix = [](uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+  };
+  uint64_t h = mix(0x9E3779B97F4A7C15ull ^ p.NUM_INPUT_POSITIONS) ^
+               mix(0xD6E8FEB86659FD93ull ^ p.NUM_INPUT_FEATURES);
+  for (const auto &e : p.in_features_index)
+    for (uint64_t snp = 0; snp < 3; ++snp)
+      h += mix(mix(e.first * 3 + snp) ^ e.second[snp]);
   return h;
 }
 
@@ -588,9 +595,9 @@ class RecordWriter {
     std::memcpy(buf_.data() + used_, p, n);
     used_ += n;
   }
-  void record(uint32_t index, const TLweSample *s) {
+  void record(uint32_t index, const TLweSample *s, int32_t stream_word) {
     raw(&index, 4);
-    raw(&kSampleStreamTag, 4);
+    raw(&stream_word, 4);
     raw(s->a, IdashParams::N * 4);
     raw(s->b, IdashParams::N * 4);
     raw(s->current_variance, 8);
@@ -608,6 +615,8 @@ class RecordWriter {
 };
 
 // Reads `count` records into a dense block of `slots` samples (slot = index).
+// Slots no record fills are zeroed. Every record of a file must open its
+// sample stream with the same framing word, which the block keeps.
 template <typename Map>
 std::shared_ptr<SampleBlock> read_records(const string &filename, size_t slots, Map &map,
                                           const char *what) {
@@ -617,19 +626,38 @@ std::shared_ptr<SampleBlock> read_records(const string &filename, size_t slots, 
   REQUIRE_DRAMATICALLY(std::fread(&count, 8, 1, f) == 1, "truncated " << what << " file");
   auto block = std::make_shared<SampleBlock>(slots);
   std::vector<char> rec(4 + kSampleStreamBytes);
+  std::vector<char> filled(slots, 0);
   for (uint64_t i = 0; i < count; ++i) {
     REQUIRE_DRAMATICALLY(std::fread(rec.data(), rec.size(), 1, f) == 1, "truncated " << what << " file");
     uint32_t index;
+    int32_t word;
     std::memcpy(&index, rec.data(), 4);
+    std::memcpy(&word, rec.data() + 4, 4);
     REQUIRE_DRAMATICALLY(index < slots, what << " index out of range");
+    if (i == 0) block->stream_word = word;
+    REQUIRE_DRAMATICALLY(word == block->stream_word,
+                         what << " records open with different framing words");
     TLweSample *s = block->sample(index);
     REQUIRE_DRAMATICALLY(map.emplace(index, s).second, "duplicate " << what << " index");
+    filled[index] = 1;
     std::memcpy(s->a, rec.data() + 8, IdashParams::N * 4);
     std::memcpy(s->b, rec.data() + 8 + IdashParams::N * 4, IdashParams::N * 4);
     std::memcpy(s->current_variance, rec.data() + 8 + kCtBytes, 8);
   }
   std::fclose(f);
+  for (size_t i = 0; i < slots; ++i)
+    if (!filled[i]) {
+      TLweSample *s = block->sample(i);
+      std::memset(s->a, 0, kCtBytes);
+      *s->current_variance = 0.0;
+    }
   return block;
+}
+
+// The framing word a collection's samples are written with: the word of the
+// block they were read into or computed from (0 for blocks encrypted here).
+int32_t stream_word_of(const std::shared_ptr<SampleBlock> &block) {
+  return block ? block->stream_word : 0;
 }
 
 size_t num_input_cts(const IdashParams &p) {
@@ -644,7 +672,8 @@ void write_encrypted_data(const EncryptedData &encrypted_data, const IdashParams
   REQUIRE_DRAMATICALLY(w.ok(), "Cannot open encrypted data file for write");
   const uint64_t n = encrypted_data.enc_data.size();
   w.raw(&n, 8);
-  for (const auto &e : encrypted_data.enc_data) w.record(e.first, e.second);
+  const int32_t word = stream_word_of(encrypted_data.block);
+  for (const auto &e : encrypted_data.enc_data) w.record(e.first, e.second, word);
 }
 
 void read_encrypted_data(EncryptedData &encrypted_data, const IdashParams &params,
@@ -659,7 +688,8 @@ void write_encrypted_predictions(const EncryptedPredictions &encrypted_preds, co
   REQUIRE_DRAMATICALLY(w.ok(), "Cannot open encrypted predictions file for write");
   const uint64_t n = encrypted_preds.score.size();
   w.raw(&n, 8);
-  for (const auto &e : encrypted_preds.score) w.record(e.first, e.second);
+  const int32_t word = stream_word_of(encrypted_preds.block);
+  for (const auto &e : encrypted_preds.score) w.record(e.first, e.second, word);
 }
 
 void read_encrypted_predictions(EncryptedPredictions &encrypted_preds, const IdashParams &params,
@@ -820,10 +850,23 @@ void cloud_compute_score(EncryptedPredictions &enc_preds, const EncryptedData &e
   // outputs are created once each (eval/idash.cpp:777-779)
   for (const auto &e : model.model)
     if (enc_preds.score.count(e.first)) DIE_DRAMATICALLY("output ciphertext created twice");
-
+  // every ciphertext the model reads must be present (getTLWE,
+  // eval/idash.h:162-166), checked before the device is touched
   const size_t cts = num_input_cts(params);
+  {
+    std::vector<char> present(cts, 0);
+    for (const auto &e : enc_data.enc_data)
+      if (e.first < cts) present[e.first] = 1;
+    for (uint32_t b : in_bidx)
+      if (b != params.constant_bigIndex()) {
+        const FeatIndex f = params.feature_indexOf(b);
+        REQUIRE_DRAMATICALLY(f < cts && present[f], "missing input ciphertext");
+      }
+  }
+
   std::shared_ptr<SampleBlock> in = dense_view(enc_data.enc_data, enc_data.block, cts);
   auto out = std::make_shared<SampleBlock>(num_out);
+  out->stream_word = stream_word_of(enc_data.block);
   if (num_out) {
     ihe_ctx *ctx = engine();
     const ihe_layout lay = layout_of(params);

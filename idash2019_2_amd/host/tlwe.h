@@ -55,6 +55,13 @@ class SampleBlock {
   double *variances() { return var_; }
   const double *variances() const { return var_; }
 
+  // The 4-byte word that opens each sample's stream in a ciphertext file
+  // (written by TFHE's export_tlweSample_toStream before a, b and the
+  // variance). Files read keep the word their records carried, and samples
+  // computed from them (the cloud step's outputs) are written with it; blocks
+  // this build encrypts carry 0.
+  int32_t stream_word = 0;
+
  private:
   size_t count_;
   int32_t *words_ = nullptr;

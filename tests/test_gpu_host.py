@@ -119,3 +119,30 @@ def test_cli_decrypt_by_position(tmp_path):
     assert lines[0] == "Subject ID,target SNP,0,1,2"
     assert lines[1].split(",")[1] == str(int(c.target_pos[0]))
     assert len(lines) == 2 + 64 * 40
+
+
+def test_cloud_passes_the_framing_word_through(tmp_path):
+    """Records read with a framing word other than this build's 0 (a file from
+    another TLWE implementation) keep it: the cloud step writes the word its
+    inputs carried on every prediction record, and the predictions decrypt as
+    they do with the word 0."""
+    c = synth.generate(40, 80, 128, 5, seed=21)
+    synth.write_reference_files(c, str(tmp_path))
+    d = str(tmp_path)
+    run("keygen", "targets.txt", "tags.txt", "1", cwd=d)
+    run("encrypt", "tags.txt", cwd=d)
+    run("cloud", "model", cwd=d)
+    run("decrypt", cwd=d)
+    plain = open(os.path.join(d, "result.csv")).read()
+    pred0 = formats.read_records(os.path.join(d, "encrypted_prediction.bin"))
+    assert (formats.read_stream_words(os.path.join(d, "encrypted_prediction.bin")) == 0).all()
+    path = os.path.join(d, "encrypted_data.bin")
+    idx, ct, var = formats.read_records(path)
+    formats.write_records(path, idx, ct, var, stream_word=0x5A17F00D - (1 << 32))
+    run("cloud", "model", cwd=d)
+    words = formats.read_stream_words(os.path.join(d, "encrypted_prediction.bin"))
+    assert words.size == c.num_outputs and (words == 0x5A17F00D - (1 << 32)).all()
+    pred1 = formats.read_records(os.path.join(d, "encrypted_prediction.bin"))
+    assert (pred0[1] == pred1[1]).all() and (pred0[2] == pred1[2]).all()
+    run("decrypt", cwd=d)
+    assert open(os.path.join(d, "result.csv")).read() == plain

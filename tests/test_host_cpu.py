@@ -148,6 +148,69 @@ def test_model_cache_is_written_validated_and_refreshed(tmp_path):
     run("cloud", "model", cwd=tmp_path)
     assert read_model_cache(cache)[(pos, 1)][1][0xFFFFFFFF] == 4321
     assert p["NUM_OUTPUT_FEATURES"] == 3 * c.num_targets
+    # a params file that changes only the middle big index of two tags (the .hr
+    # files untouched): the cache no longer matches, and the model is re-read
+    # with the new index
+    raw = bytearray((tmp_path / "params.bin").read_bytes())
+    q0, q1 = int(c.lo[0]), int(c.lo[0]) + 1          # two tags of target 0's window
+    tp = c.tag_pos.tolist()
+    # entries are (uint64 pos, 3 x uint32) in the file's own order
+    entry = {int.from_bytes(raw[28 + 20 * i:36 + 20 * i], "little"): 28 + 20 * i
+             for i in range(c.num_tags)}
+    off = lambda q, v: entry[tp[q]] + 8 + 4 * v
+    b0 = raw[off(q0, 1):off(q0, 1) + 4]
+    raw[off(q0, 1):off(q0, 1) + 4] = raw[off(q1, 1):off(q1, 1) + 4]
+    raw[off(q1, 1):off(q1, 1) + 4] = b0
+    (tmp_path / "params.bin").write_bytes(bytes(raw))
+    p2 = formats.read_params(str(tmp_path / "params.bin"))
+    assert p2["in_features_index"][tp[q0]][1] == 3 * q1 + 1
+    run("cloud", "model", cwd=tmp_path)
+    ent = read_model_cache(cache)[(c.target_pos.tolist()[0], 0)][1]
+    j0, j1 = 3 * (q0 - int(c.lo[0])) + 1, 3 * (q1 - int(c.lo[0])) + 1
+    assert ent[3 * q1 + 1] == int(c.weights[0, j0])  # tag q0's "_1" column now maps there
+    assert ent[3 * q0 + 1] == int(c.weights[0, j1])
+
+
+def write_inputs(tmp_path, c, keep, stream_word=0):
+    """encrypted_data.bin with random ciphertexts at the FeatIndex values `keep`."""
+    rng = np.random.default_rng(1)
+    ct = rng.integers(-(1 << 31), 1 << 31, (c.num_in_cts, 2, 1024), dtype=np.int64).astype(np.int32)
+    var = np.full(c.num_in_cts, 2.0 ** -50)
+    formats.write_records(str(tmp_path / "encrypted_data.bin"), keep, ct, var, stream_word)
+
+
+def test_cloud_dies_on_a_missing_input_record(tmp_path):
+    """A model coefficient whose ciphertext the encrypted-data file lacks ends
+    the cloud step with ERROR before any device work (getTLWE,
+    eval/idash.h:162-166), whatever the file's length."""
+    c = synth.generate(20, 30, 64, 3, seed=9)        # R = 16: 4 input ciphertexts
+    synth.write_reference_files(c, str(tmp_path))
+    assert run("keygen", "targets.txt", "tags.txt", "1", cwd=tmp_path).returncode == 0
+    R = c.layout[1]
+    used = sorted({(3 * int(c.lo[t]) + j) // R for t in range(c.num_targets)
+                   for j in range(3 * c.neighbours)})
+    drop = used[len(used) // 2]
+    write_inputs(tmp_path, c, [i for i in range(c.num_in_cts) if i != drop])
+    r = run("cloud", "model", cwd=tmp_path)
+    assert r.returncode != 0 and "ERROR: missing input ciphertext" in r.stdout
+    assert not (tmp_path / "encrypted_prediction.bin").exists()
+
+
+def test_input_records_with_different_framing_words_are_rejected(tmp_path):
+    """Every record of a file opens its sample stream with one framing word;
+    a file mixing two is rejected when read."""
+    c = synth.generate(20, 30, 64, 3, seed=9)
+    synth.write_reference_files(c, str(tmp_path))
+    assert run("keygen", "targets.txt", "tags.txt", "1", cwd=tmp_path).returncode == 0
+    write_inputs(tmp_path, c, list(range(c.num_in_cts)), stream_word=0x1234)
+    path = tmp_path / "encrypted_data.bin"
+    raw = bytearray(path.read_bytes())
+    raw[8 + formats.RECORD_BYTES + 4:8 + formats.RECORD_BYTES + 8] = (7).to_bytes(4, "little")
+    path.write_bytes(bytes(raw))
+    assert formats.read_stream_words(str(path)).tolist()[:2] == [0x1234, 7]
+    r = run("cloud", "model", cwd=tmp_path)
+    assert r.returncode != 0
+    assert "ERROR: encrypted data records open with different framing words" in r.stdout
 
 
 def test_sanitizer_build_recipe():
