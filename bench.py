@@ -7,12 +7,15 @@ ciphertexts already resident in HBM: 16,184 tag SNPs, 80,882 target SNPs,
 The inputs are produced on the device by the library's own encrypt kernel.
 
 Prints one JSON line. `value` is imputed target SNPs per second
-(targets x samples / step time). `roofline` prices the main cloud kernel
-against HBM bandwidth with its algorithmic bytes (every input ciphertext read
-once, every output written once, plus the band); its launch durations come
-from HIP events on the launch stream. `cpu_baseline` times the CPU
-restatement of the reference's cloud step (hash-map model, OpenMP over
-outputs) on this host.
+(targets x samples / step time). A step is one launch: the cloud kernel with
+its variance blocks in the same grid. `roofline` prices that launch against
+HBM bandwidth with its algorithmic bytes (every input ciphertext read once,
+every output written once, plus the band); its duration is the time between
+two HIP events on the launch stream around the timed steps, divided by the
+steps, and `roofline.kernel_only` gives the same from a separate pass with
+an event pair around each launch. `cpu_baseline` times the CPU restatement of
+the reference's cloud step (hash-map model, OpenMP over outputs) on this
+host.
 """
 from __future__ import annotations
 
@@ -32,6 +35,7 @@ from idash2019_2_amd import Engine, Server, _abi, synth  # noqa: E402
 METRIC = ("encrypted target SNPs imputed/s at 1 GPU; % HBM roofline; "
           "p50/p99 cohort latency")
 HBM_PEAK_GBS = 8000.0
+CPU_SHARE = 16          # host cores that go with one GPU of the box
 VARIANTS = {"auto": _abi.VARIANT_AUTO, "valu": _abi.VARIANT_VALU, "wide": _abi.VARIANT_WIDE,
             "mfma": _abi.VARIANT_MFMA}
 
@@ -40,15 +44,35 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def source_hash() -> str:
+    """SHA-256 (first 16 hex digits) of the GPU library's sources: the kernels,
+    their headers, the build flags and the C ABI header."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "idash2019_2_amd", "csrc")
+    files = sorted(f for f in os.listdir(csrc) if f.endswith((".hip", ".h")) or f == "Makefile")
+    for f in files + ["../../include/idash_hip.h"]:
+        h.update(f.encode())
+        with open(os.path.join(csrc, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def read_traffic(workload: str, variant: str):
-    """HBM bytes per launch measured with rocprofv3 counters, if a summary is committed."""
+    """HBM bytes per launch measured with rocprofv3 counters, from the committed
+    summary, when it was taken on the library sources of this run (its
+    source_hash line); None otherwise."""
     path = os.path.join(ROOT, "profiles", f"cloud_{variant}_{workload}_traffic.txt")
     if not os.path.exists(path):
         return None
+    fields = {}
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    if fields.get("source_hash") != source_hash() or "traffic_bytes_per_launch" not in fields:
+        return None
+    return float(fields["traffic_bytes_per_launch"])
 
 
 def build_bands(eng, models, keys, num_in, shapes=None):
@@ -194,36 +218,32 @@ def forms_served(eng, base, log_fn, seed=5, num_requests=64, budget=64 << 30, sl
                 tail, a.shape)
 
     def run(nslots, arrivals):
+        """Submits every request at its scheduled arrival (back to back when
+        arrivals is None) and waits for all; each latency runs from the request's
+        scheduled arrival (its submit, back to back) to its scores on the host, as
+        the server clocks it. The scores are digested after the timed run."""
         srv = eng.server(budget, num_slots=nslots)
         mids = [srv.add_model(bands[(s, 5)], base.num_tags) for s in range(3)]
-        tickets, outs, lat, dig = [], {}, {}, {}
-        t0 = time.perf_counter()
-        pending = []
+        tickets = []
+        t0 = time.monotonic_ns()
         for i, (m, geno, key, seed_i, ns_i) in enumerate(reqs):
+            due = 0
             if arrivals is not None:
-                while time.perf_counter() - t0 < arrivals[i]:
+                due = t0 + int(arrivals[i] * 1e9)
+                while time.monotonic_ns() < due:
                     time.sleep(0.0002)
-            out = pinned[i].array
-            outs[i] = out
-            pending.append((i, srv.submit(mids[m], geno, key, seed_i, out, num_samples=ns_i)))
-            still = []
-            for j, t in pending:
-                ms = srv.poll(t)
-                if ms is None:
-                    still.append((j, t))
-                else:
-                    lat[j] = ms
-                    dig[j] = digest(outs.pop(j))
-            pending = still
-        for j, t in pending:
-            lat[j] = srv.wait(t)
-            dig[j] = digest(outs.pop(j))
-            if len(dig) % 16 == 0:
-                log_fn(f"served: {len(dig)} of {len(reqs)} requests done")
-        elapsed = time.perf_counter() - t0
+            tickets.append(srv.submit(mids[m], geno, key, seed_i, pinned[i].array,
+                                      num_samples=ns_i, arrival_ns=due))
+        lat = []
+        for k, t in enumerate(tickets):
+            lat.append(srv.wait(t))
+            if (k + 1) % 16 == 0:
+                log_fn(f"served: {k + 1} of {len(reqs)} requests done")
+        elapsed = (time.monotonic_ns() - t0) / 1e9
         peak = srv.stats()["arena_peak"]
         srv.close()
-        return elapsed, [lat[i] for i in range(len(reqs))], [dig[i] for i in range(len(reqs))], peak
+        dig = [digest(p.array) for p in pinned]
+        return elapsed, lat, dig, peak
 
     log_fn(f"served form: {len(reqs)} requests, {sum(int(x) for x in ns_list)} samples in all")
     run(slots, None)                                       # warm-up: first DMA to the buffers
@@ -400,28 +420,53 @@ def main(argv=None):
         for _ in range(args.warmup):
             eng.cloud(band, d_ct, d_var, d_out, d_ov)
         eng.sync()
-        eng.timing(True)
+        eng.timing(False)
         eng.timing_reset()
-        eng.sync()
+        # the timed steps: synchronized on both sides, bracketed on the launch
+        # stream by two events and nothing between the launches
         t_start = time.perf_counter()
+        eng.timing_region(True)
         for _ in range(args.steps):
             eng.cloud(band, d_ct, d_var, d_out, d_ov)
+        eng.timing_region(False)
         eng.sync()
         elapsed = time.perf_counter() - t_start
+        region_ms, _ = eng.timing_read(_abi.TIMER_REGION)
+        # a separate pass of the same steps with an event pair around each launch
+        eng.timing(True)
+        eng.timing_reset()
+        for _ in range(args.steps):
+            eng.cloud(band, d_ct, d_var, d_out, d_ov)
         kern_ms, kern_n = eng.timing_read(_abi.TIMER_CLOUD)
-        var_ms, _ = eng.timing_read(_abi.TIMER_CLOUD_VAR)
+        var_ms, var_n = eng.timing_read(_abi.TIMER_CLOUD_VAR)
         eng.timing(False)
-        log(f"{args.steps} steps in {elapsed * 1e3:.2f} ms; cloud kernel avg "
-            f"{kern_ms / kern_n:.4f} ms, variance kernel avg {var_ms / kern_n:.4f} ms")
+        launches = (kern_n + var_n) / args.steps
+        log(f"{args.steps} steps in {elapsed * 1e3:.2f} ms wall, {region_ms:.3f} ms between the "
+            f"region's events; {launches:g} launch(es) per step; cloud launch avg "
+            f"{kern_ms / kern_n:.4f} ms" + (f", variance launch avg {var_ms / var_n:.4f} ms"
+                                           if var_n else " (variances in the same launch)"))
 
         step_s = elapsed / args.steps
         value = c.num_targets * ns / step_s
         alg_bytes = synth.cloud_bytes(c)
+        step_event_s = region_ms / args.steps / 1e3
         avg_kernel_s = kern_ms / kern_n / 1e3
-        achieved = alg_bytes / avg_kernel_s / 1e9
+        achieved = alg_bytes / step_event_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": read_traffic(args.workload, variant_name)}
+                    "traffic": read_traffic(args.workload, variant_name),
+                    "duration": "average step between two HIP events on the launch stream around "
+                                "the timed steps (one launch per step: the cloud kernel with its "
+                                "variance blocks)" if launches == 1 else
+                                "average step between two HIP events on the launch stream around "
+                                "the timed steps",
+                    "launches_per_step": launches,
+                    "kernel_only": {"ms": round(avg_kernel_s * 1e3, 4),
+                                    "achieved": round(alg_bytes / avg_kernel_s / 1e9, 1),
+                                    "frac": round(alg_bytes / avg_kernel_s / 1e9 / HBM_PEAK_GBS, 4),
+                                    "timed": "a separate pass of the same steps, an event pair "
+                                             "around each cloud launch"},
+                    "source_hash": source_hash()}
 
         # the bandwidth HBM sustains for the same read and write volumes, from a
         # contiguous streaming kernel (the attainable ceiling, beside the spec peak)
@@ -493,10 +538,12 @@ def main(argv=None):
 
         cpu = None
         if not args.no_cpu_baseline:
-            threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) \
-                or min(16, os.cpu_count() or 1)
+            # NB_THREADS = the CPUs this process may run on (eval/idash.h:36-38 makes it a
+            # build constant), held to the host's share for one GPU (CPU_SHARE cores)
+            visible = len(os.sched_getaffinity(0))
+            threads = args.cpu_threads or min(visible, CPU_SHARE)
             secs, same, secs4 = cpu_baseline(eng, d_ct, c, rp, ib, w, d_out, threads)
-            log(f"cpu baseline: {secs:.3f} s with {threads} threads"
+            log(f"cpu baseline: {secs:.3f} s with {threads} threads ({visible} CPUs visible)"
                 + (f", {secs4:.3f} s with 4" if secs4 else "")
                 + f"; GPU output {'equals' if same else 'DIFFERS FROM'} the CPU output")
             if not same:
@@ -505,6 +552,9 @@ def main(argv=None):
                    "cores": threads, "kind": "port",
                    "sample": f"one full cloud step of the same workload ({c.num_targets} targets "
                              f"x {ns} samples), compute only, as eval/cloud.cpp:16-18 times it",
+                   "cpus_visible": visible,
+                   "threads_rule": f"OpenMP threads = min(CPUs visible, {CPU_SHARE}), the host's "
+                                   "CPU share for one GPU",
                    "cpu_model": cpu_model()}
             if secs4:
                 cpu["value_at_4_threads"] = round(c.num_targets * ns / secs4, 1)
@@ -533,9 +583,10 @@ def main(argv=None):
                    "algorithmic_bytes": alg_bytes,
                    "target_positions_per_s": round(c.num_targets / step_s, 1),
                    "cloud_kernel_ms": round(avg_kernel_s * 1e3, 4),
+                   "step_event_ms": round(step_event_s * 1e3, 4),
                    "step_with_host_copies_ms": round(pcie_s * 1e3, 3),
                    "value_with_host_copies": round(c.num_targets * ns / pcie_s, 1),
-                   "variance_kernel_ms": round(var_ms / kern_n, 4),
+                   "variance_kernel_ms": round(var_ms / var_n, 4) if var_n else None,
                    "encrypt_kernel_ms": round(enc_ms / enc_n, 4),
                    "decrypt_kernel_ms": round(dec_ms / dec_n, 4)},
         "roofline": roofline,

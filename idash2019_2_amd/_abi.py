@@ -24,7 +24,8 @@ IHE_CONSTANT_BIDX = 0xFFFFFFFF
 IHE_NOISE_TABLE_LEN = 1280
 
 VARIANT_AUTO, VARIANT_VALU, VARIANT_WIDE, VARIANT_MFMA = 0, 1, 2, 3
-TIMER_CLOUD, TIMER_CLOUD_VAR, TIMER_ENCRYPT, TIMER_DECRYPT, TIMER_BATCH, TIMER_STREAM = range(6)
+(TIMER_CLOUD, TIMER_CLOUD_VAR, TIMER_ENCRYPT, TIMER_DECRYPT, TIMER_BATCH, TIMER_STREAM,
+ TIMER_REGION) = range(7)
 
 # Every entry point the header declares, with (restype, argtypes).
 _vp, _u32, _i32, _u64, _sz, _int = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32,
@@ -61,7 +62,7 @@ class ServerConfig(ctypes.Structure):
 
 class Request(ctypes.Structure):
     _fields_ = [("model", _u32), ("num_samples", _u32), ("geno", _vp), ("key", _vp),
-                ("seed", _u64), ("out", _vp)]
+                ("seed", _u64), ("out", _vp), ("arrival_ns", _u64)]
 
 
 class ServerStats(ctypes.Structure):
@@ -98,6 +99,7 @@ SIGNATURES = {
     "ihe_stream_probe": (_int, [_vp, _vp, _sz, _vp, _sz]),
     "ihe_timing_enable": (_int, [_vp, _int]),
     "ihe_timing_reset": (_int, [_vp]),
+    "ihe_timing_region": (_int, [_vp, _int]),
     "ihe_timing_read": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(_u32)]),
     "ihe_band_create": (_int, [_vp, ctypes.POINTER(Layout), _u32, ctypes.POINTER(ModelCSR), _int,

@@ -2,6 +2,7 @@
 // (ihe_cloud.hip) and the batched launcher (ihe_batch.hip).
 #pragma once
 
+#include <atomic>
 #include <vector>
 
 #include "ihe_mfma.h"
@@ -23,6 +24,8 @@ enum { kModeDirect = 0, kModeStaged = 1, kModeMfma = 2 };
 
 struct ihe_band {
   ihe_ctx *ctx = nullptr;
+  // batches and servers holding the band (ihe_band_destroy refuses while > 0)
+  mutable std::atomic<uint32_t> users{0};
   ihe_layout layout{};
   uint32_t num_in = 0, num_out = 0;
   int variant = IHE_VARIANT_VALU;

@@ -25,6 +25,7 @@ struct ihe_batch {
     uint32_t entry0, num_entries, first0, num_items;
   };
   std::vector<Launch> launches;
+  std::vector<const ihe_band *> bands;  // each distinct band once; its use count is held
   uint32_t num_cohorts = 0, num_items = 0;
   uint32_t num_ventries = 0, num_vblocks = 0;
   void *dmem = nullptr;
@@ -53,13 +54,18 @@ bool same_family(const Keyed &a, const Keyed &b) {
 
 void enqueue(const ihe_batch *b, hipStream_t s) {
   MfmaEntry none = {};
-  for (const ihe_batch::Launch &L : b->launches)
+  VarArgs V{};
+  V.entries = b->d_ventries;
+  V.first = b->d_vfirst;
+  V.num_entries = b->num_ventries;
+  V.num_blocks = b->num_ventries ? b->num_vblocks : 0;
+  // the variance blocks join the last family's launch
+  for (size_t i = 0; i < b->launches.size(); ++i) {
+    const ihe_batch::Launch &L = b->launches[i];
     launch_mfma(s, L.shape, L.rot, none, b->d_entries + L.entry0, b->d_first + L.first0,
-                L.num_entries, L.num_items);
-  if (b->num_ventries) {
-    VarEntry vnone = {};
-    launch_var(s, vnone, b->d_ventries, b->d_vfirst, b->num_ventries, b->num_vblocks);
+                L.num_entries, L.num_items, i + 1 == b->launches.size() ? V : VarArgs{});
   }
+  if (b->launches.empty()) launch_var(s, V);
 }
 
 }  // namespace
@@ -180,6 +186,10 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
     delete b;
     return rc;
   }
+  for (uint32_t i = 0; i < num_cohorts; ++i)
+    if (std::find(b->bands.begin(), b->bands.end(), cohorts[i].band) == b->bands.end())
+      b->bands.push_back(cohorts[i].band);
+  for (const ihe_band *band : b->bands) band->users++;
   *out = b;
   return IHE_OK;
 }
@@ -214,7 +224,7 @@ int ihe_batch_capture(ihe_batch *b) {
 int ihe_batch_get_info(const ihe_batch *b, ihe_batch_info *info) {
   if (!b || !info) return fail(IHE_ERR_ARG, "null argument");
   info->num_cohorts = b->num_cohorts;
-  info->num_launches = (uint32_t)b->launches.size() + (b->num_ventries ? 1u : 0u);
+  info->num_launches = (uint32_t)b->launches.size() + (b->launches.empty() && b->num_ventries ? 1u : 0u);
   info->num_tiles = b->num_items;
   info->captured = b->exec ? 1u : 0u;
   return IHE_OK;
@@ -226,6 +236,7 @@ int ihe_batch_destroy(ihe_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   (void)ihe_free(b->ctx, b->dmem);
+  for (const ihe_band *band : b->bands) band->users--;
   delete b;
   return IHE_OK;
 }

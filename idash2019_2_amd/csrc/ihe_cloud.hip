@@ -1152,7 +1152,7 @@ int ihe_band_plan(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model
 namespace {
 
 void launch_mfma_band(ihe_ctx *ctx, const ihe_band *band, const ihe_layout &L, uint32_t num_in,
-                      const int32_t *d_in, int32_t *d_out);
+                      const int32_t *d_in, int32_t *d_out, const VarArgs &var = VarArgs{});
 
 // Uploads a compiled band.
 int upload_band(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_in_cts,
@@ -1313,6 +1313,9 @@ int ihe_band_create_tuned(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_i
 
 int ihe_band_destroy(ihe_band *band) {
   if (!band) return IHE_OK;
+  if (band->users.load() > 0)
+    return fail(IHE_ERR_BUSY, "the band is used by " + std::to_string(band->users.load()) +
+                                  " live batch or server holder(s)");
   (void)ihe_sync(band->ctx);
   (void)ihe_free(band->ctx, band->dmem);
   delete band;
@@ -1401,9 +1404,10 @@ void launch_staged(ihe_ctx *ctx, const ihe_band *band, const int32_t *d_in, int3
   }
 }
 
-// One launch per segment; the tiling is shared by every layout.
+// One launch per segment; the tiling is shared by every layout. The variance
+// blocks (if any) join the last segment's grid.
 void launch_mfma_band(ihe_ctx *ctx, const ihe_band *band, const ihe_layout &L, uint32_t num_in,
-                      const int32_t *d_in, int32_t *d_out) {
+                      const int32_t *d_in, int32_t *d_out, const VarArgs &var) {
   for (const Segment &s : band->segments) {
     MfmaEntry E = {};
     E.tiles = band->d_mtiles;
@@ -1421,9 +1425,20 @@ void launch_mfma_band(ihe_ctx *ctx, const ihe_band *band, const ihe_layout &L, u
     E.out = d_out + (size_t)s.out_base * kCtWords;
     E.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
     E.tile_base = s.tile_begin;
+    const bool last = &s == &band->segments.back();
     launch_mfma(ctx->stream, band->shape, L.num_regions > 1, E, nullptr, nullptr, 0,
-                s.tile_end - s.tile_begin);
+                s.tile_end - s.tile_begin, last ? var : VarArgs{});
   }
+}
+
+// The variance work of one cohort on a band.
+VarArgs band_variances(const ihe_band *band, const ihe_layout &L, const double *d_in_var,
+                       double *d_out_var) {
+  VarArgs V{};
+  V.single = make_var_entry(band->d_var_ptr, band->d_var_col, band->d_var_w2, band->d_var_base,
+                            d_in_var, d_out_var, band->num_out, L.num_regions);
+  V.num_blocks = (band->num_out + 255) / 256;
+  return V;
 }
 
 }  // namespace
@@ -1448,19 +1463,21 @@ int ihe_cloud_score_layout(ihe_ctx *ctx, const ihe_band *band, const ihe_layout 
   if (!d_out || (num_in_cts > 0 && !d_in)) return fail(IHE_ERR_ARG, "null ciphertext buffer");
   if (d_out_var && num_in_cts > 0 && !d_in_var) return fail(IHE_ERR_ARG, "null input variance");
   if (int rc = make_current(ctx)) return rc;
+  // MFMA bands compute the variances in the cloud launch itself (its trailing
+  // blocks); IHE_CLOUD_FUSED_VAR=0 gives them a launch of their own
+  const bool fused = band->mode == kModeMfma && env_u32("IHE_CLOUD_FUSED_VAR", 1, 0, 1) == 1;
+  const VarArgs V = d_out_var ? band_variances(band, L, d_in_var, d_out_var) : VarArgs{};
   {
     ScopedTimer t(ctx, IHE_TIMER_CLOUD);
-    if (band->mode == kModeMfma) launch_mfma_band(ctx, band, L, num_in_cts, d_in, d_out);
+    if (band->mode == kModeMfma)
+      launch_mfma_band(ctx, band, L, num_in_cts, d_in, d_out, fused ? V : VarArgs{});
     else if (band->mode == kModeStaged) launch_staged(ctx, band, d_in, d_out);
     else launch_direct(ctx, band, d_in, d_out);
   }
   IHE_HIP(hipGetLastError());
-  if (d_out_var) {
+  if (d_out_var && !fused) {
     ScopedTimer t(ctx, IHE_TIMER_CLOUD_VAR);
-    const VarEntry V = make_var_entry(band->d_var_ptr, band->d_var_col, band->d_var_w2,
-                                      band->d_var_base, d_in_var,
-                                      d_out_var, band->num_out, L.num_regions);
-    launch_var(ctx->stream, V, nullptr, nullptr, 0, (band->num_out + 255) / 256);
+    launch_var(ctx->stream, V);
     IHE_HIP(hipGetLastError());
   }
   return IHE_OK;

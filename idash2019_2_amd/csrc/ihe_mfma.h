@@ -63,13 +63,6 @@ inline uint32_t mfma_wg_per_tile(const MfmaShape &s) {
   return kCtWords / (waves_per_tile * (uint32_t)s.ntw * 32);
 }
 
-// Enqueues the MFMA cloud kernel. Single cohort: entries == nullptr and
-// `single` describes it, with num_items = its tiles. Batch: entries[e] for
-// e < num_entries, first[e] = index of entry e's first tile in the launch
-// (first[num_entries] = num_items).
-void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const MfmaEntry &single,
-                 const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
-                 uint32_t num_items);
 
 // Output variances: var_out[o] = sum over the row's entries with c % R == 0 of
 // w2 * var_in[c / R] (int32 w*w, summed in row order). Entries are stored
@@ -99,7 +92,28 @@ inline VarEntry make_var_entry(const uint32_t *blk_off, const uint32_t *col, con
 }
 // A packed entry's offset field for padding.
 constexpr uint32_t kVarPackedPad = 0xFFFFu;
-void launch_var(hipStream_t stream, const VarEntry &single, const VarEntry *entries,
-                const uint32_t *first, uint32_t num_entries, uint32_t num_blocks);
+
+// Variance work of a launch: one cohort (`entries` null: `single`) or a table
+// of cohorts (first[e] = entry e's first block, first[num_entries] =
+// num_blocks), in blocks of 256 outputs.
+struct VarArgs {
+  VarEntry single;
+  const VarEntry *entries;
+  const uint32_t *first;
+  uint32_t num_entries;
+  uint32_t num_blocks;
+};
+
+void launch_var(hipStream_t stream, const VarArgs &var);
+
+// Enqueues the MFMA cloud kernel. Single cohort: entries == nullptr and
+// `single` describes it, with num_items = its tiles. Batch: entries[e] for
+// e < num_entries, first[e] = index of entry e's first tile in the launch
+// (first[num_entries] = num_items). `var` (num_blocks > 0) appends its
+// variance blocks to the same grid: the step is one launch, and the variance
+// blocks fill the cloud blocks' drain tail.
+void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const MfmaEntry &single,
+                 const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
+                 uint32_t num_items, const VarArgs &var);
 
 }  // namespace ihe

@@ -46,7 +46,11 @@ struct MfmaArgs {
   const MfmaEntry *entries;
   const uint32_t *first;
   uint32_t num_entries;
+  uint32_t cloud_blocks;  // blocks at or past this index run variance blocks
+  VarArgs var;
 };
+
+__device__ __forceinline__ void var_block(const VarArgs &A, uint32_t chunk);
 
 struct EntryWords {
   u32x16v lo, hi;
@@ -214,7 +218,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfma1(MfmaArgs A) {
   constexpr uint32_t wg_per_tile = kCtWords / (kWaves * NTW * 32);
   constexpr uint32_t rows = kMfmaRows * RB;
   __shared__ uint32_t s_bias[rows];
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
+  const uint32_t blk = xcd_block(blockIdx.x, A.cloud_blocks);
   uint32_t tile;
   const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
   const uint32_t part = blk % wg_per_tile;
@@ -287,7 +292,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
   __shared__ uint32_t s_cw[NKB * 32];
   __shared__ int4v s_a[NKB * 2 * 64];
   __shared__ uint32_t s_bias[kMfmaRows];
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
+  const uint32_t blk = xcd_block(blockIdx.x, A.cloud_blocks);
   uint32_t tile;
   const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
   const uint32_t part = blk % wg_per_tile;
@@ -401,7 +407,8 @@ void k_cloud_mfmaw(MfmaArgs A) {
   constexpr int kRounds = kFull + (NKB % 4 ? 1 : 0);
   __shared__ int4v s_b[2][NKB * 4 * 64];  // [buffer][(kb * 4 + plane) * 64 + B lane]
   __shared__ uint32_t s_bias[rows];
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
+  const uint32_t blk = xcd_block(blockIdx.x, A.cloud_blocks);
   uint32_t tile;
   const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
   const uint32_t nt0 = (blk % wg_per_tile) * NTW;
@@ -519,13 +526,6 @@ void k_cloud_mfmaw(MfmaArgs A) {
 
 // ---------------------------------------------------------------------------
 // Output variances, 256 outputs per block.
-struct VarArgs {
-  VarEntry single;
-  const VarEntry *entries;
-  const uint32_t *first;
-  uint32_t num_entries;
-};
-
 __device__ __forceinline__ uint32_t div_r(uint32_t c, uint32_t R, uint64_t rmagic) {
   return R == 1 ? c : (uint32_t)__umul64hi((uint64_t)c, rmagic);
 }
@@ -533,9 +533,10 @@ __device__ __forceinline__ uint32_t div_r(uint32_t c, uint32_t R, uint64_t rmagi
 // One wave per 64 outputs, lane = output. Entries are taken 8 at a time: their
 // column, weight and input-variance loads are all issued before the first
 // add, and the adds then run in entry order (the order of the model's row).
-__global__ __launch_bounds__(256) void k_cloud_var(VarArgs A) {
+// Block `chunk` of the launch's variance work; run by k_cloud_var and by the
+// cloud kernels' trailing blocks.
+__device__ __forceinline__ void var_block(const VarArgs &A, uint32_t chunk) {
   VarEntry E = A.single;
-  uint32_t chunk = blockIdx.x;
   if (A.entries) {
     uint32_t lo = 0, hi = A.num_entries;
     while (hi - lo > 1) {
@@ -597,6 +598,8 @@ __global__ __launch_bounds__(256) void k_cloud_var(VarArgs A) {
   if (o < E.num_out) E.var_out[o] = v;
 }
 
+__global__ __launch_bounds__(256) void k_cloud_var(VarArgs A) { var_block(A, blockIdx.x); }
+
 template <bool ROT, int NTW>
 void launch_k1(hipStream_t s, int rb, dim3 grid, const MfmaArgs &A) {
   if (rb == 2) hipLaunchKernelGGL((k_cloud_mfma1<ROT, NTW, 2>), grid, dim3(64 * kWaves), 0, s, A);
@@ -651,20 +654,18 @@ void launch_rot(hipStream_t s, const MfmaShape &sh, dim3 grid, const MfmaArgs &A
 
 void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const MfmaEntry &single,
                  const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
-                 uint32_t num_items) {
-  if (num_items == 0) return;
-  const uint32_t wg_per_tile = mfma_wg_per_tile(shape);
-  const MfmaArgs A = {single, entries, first, num_entries};
-  const dim3 grid(num_items * wg_per_tile);
+                 uint32_t num_items, const VarArgs &var) {
+  if (num_items == 0) return launch_var(stream, var);
+  const uint32_t cloud_blocks = num_items * mfma_wg_per_tile(shape);
+  const MfmaArgs A = {single, entries, first, num_entries, cloud_blocks, var};
+  const dim3 grid(cloud_blocks + var.num_blocks);
   if (rot) launch_rot<true>(stream, shape, grid, A);
   else launch_rot<false>(stream, shape, grid, A);
 }
 
-void launch_var(hipStream_t stream, const VarEntry &single, const VarEntry *entries,
-                const uint32_t *first, uint32_t num_entries, uint32_t num_blocks) {
-  if (num_blocks == 0) return;
-  const VarArgs A = {single, entries, first, num_entries};
-  hipLaunchKernelGGL(k_cloud_var, dim3(num_blocks), dim3(256), 0, stream, A);
+void launch_var(hipStream_t stream, const VarArgs &var) {
+  if (var.num_blocks == 0) return;
+  hipLaunchKernelGGL(k_cloud_var, dim3(var.num_blocks), dim3(256), 0, stream, var);
 }
 
 }  // namespace ihe

@@ -393,6 +393,30 @@ int ihe_timing_enable(ihe_ctx *ctx, int on) {
   return IHE_OK;
 }
 
+int ihe_timing_region(ihe_ctx *ctx, int begin) {
+  if (int rc = check_ctx(ctx)) return rc;
+  IHE_HIP(hipSetDevice(ctx->device));
+  TimerSlot &t = ctx->timers[IHE_TIMER_REGION];
+  if (begin) {
+    if (t.used == t.starts.size()) {
+      hipEvent_t a, b;
+      IHE_HIP(hipEventCreate(&a));
+      if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(IHE_ERR_HIP, "event creation failed");
+      }
+      t.starts.push_back(a);
+      t.stops.push_back(b);
+    }
+    IHE_HIP(hipEventRecord(t.starts[t.used], ctx->stream));
+    t.used++;
+    return IHE_OK;
+  }
+  if (t.used == 0) return fail(IHE_ERR_ARG, "region end without a begin");
+  IHE_HIP(hipEventRecord(t.stops[t.used - 1], ctx->stream));
+  return IHE_OK;
+}
+
 int ihe_timing_reset(ihe_ctx *ctx) {
   if (int rc = check_ctx(ctx)) return rc;
   IHE_HIP(hipSetDevice(ctx->device));

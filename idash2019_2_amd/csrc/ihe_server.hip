@@ -16,6 +16,11 @@
 // stages overlap across streams. A chunk's result depends only on its request
 // (seed + chunk index selects the random streams), so it is the same alone
 // and in the mix.
+//
+// The worker holds the server's mutex only to update the shared state
+// (requests, queue, arena, slot ownership): it polls the slots' events and
+// enqueues a chunk's copies and kernels with the mutex released, so submit,
+// poll and wait never wait behind a chunk's enqueue.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -90,17 +95,16 @@ struct Request {
   uint32_t chunks = 0, issued = 0, finished = 0;
   int rc = IHE_OK;
   std::string err;
-  Clock::time_point t_submit, t_done;
+  Clock::time_point t_arrive, t_done;
   bool done = false;
 };
 
-struct Slot {
-  ihe_ctx *ctx = nullptr;
-  hipEvent_t ev = nullptr;
-  bool busy = false;
-  uint64_t req = 0;
-  size_t off = 0, bytes = 0;
-  ihe_key *key = nullptr;
+// A chunk taken from the queue, to be enqueued with the mutex released.
+struct Issue {
+  size_t slot;
+  uint64_t id;
+  ihe_request r;
+  uint32_t k;
 };
 
 // Device bytes of one chunk: genotypes, input ciphertexts and variances,
@@ -125,6 +129,21 @@ ChunkPlan plan_chunk(const ModelRec &m, uint32_t ns) {
   return p;
 }
 
+// A slot is busy from the moment a chunk is assigned to it until the worker
+// has seen the chunk's event complete. Only the worker changes a slot, under
+// the mutex (get_stats reads `busy` under it too); the worker reads its slots
+// without the mutex.
+struct Slot {
+  ihe_ctx *ctx = nullptr;
+  hipEvent_t ev = nullptr;
+  bool busy = false;
+  bool enqueued = false;  // the chunk's work and event are on the stream
+  uint64_t req = 0;
+  size_t off = 0, bytes = 0;
+  ChunkPlan plan{};
+  ihe_key *key = nullptr;
+};
+
 }  // namespace
 
 struct ihe_server {
@@ -142,9 +161,10 @@ struct ihe_server {
   bool stop = false;
   std::thread worker;
 
-  // Enqueues chunk `k` of request `id` on slot `s` (lock held).
-  int issue(Slot &s, uint64_t id, Request &R, uint32_t k, const ChunkPlan &p) {
-    const ModelRec &m = models[R.r.model];
+  // Enqueues chunk `k` of a request on slot `s` (mutex released; the slot and
+  // its arena block are this chunk's).
+  int issue(Slot &s, const ihe_request &r, const ModelRec &m, uint32_t k) {
+    const ChunkPlan &p = s.plan;
     const uint32_t col0 = k * kChunkSamples, ns = p.L.num_samples;
     char *base = static_cast<char *>(arena_base) + s.off;
     int8_t *d_geno = reinterpret_cast<int8_t *>(base + p.o_geno);
@@ -154,28 +174,25 @@ struct ihe_server {
     float *d_dec = reinterpret_cast<float *>(base + p.o_dec);
     hipStream_t st = s.ctx->stream;
     if (int rc = make_current(s.ctx)) return rc;
-    IHE_HIP(hipMemcpyAsync(d_geno, R.r.geno + (size_t)m.num_tags * col0, (size_t)m.num_tags * ns,
+    IHE_HIP(hipMemcpyAsync(d_geno, r.geno + (size_t)m.num_tags * col0, (size_t)m.num_tags * ns,
                            hipMemcpyHostToDevice, st));
-    if (int rc = ihe_key_create(s.ctx, R.r.key, &s.key)) return rc;
-    if (int rc = ihe_encrypt(s.ctx, s.key, &p.L, m.num_tags, d_geno, R.r.seed + k, d_in, d_var))
+    if (int rc = ihe_key_create(s.ctx, r.key, &s.key)) return rc;
+    if (int rc = ihe_encrypt(s.ctx, s.key, &p.L, m.num_tags, d_geno, r.seed + k, d_in, d_var))
       return rc;
     if (int rc = ihe_cloud_score_layout(s.ctx, m.band, &p.L, p.num_in, d_in, nullptr, d_out, nullptr))
       return rc;
     if (int rc = ihe_decrypt(s.ctx, s.key, ns, d_out, m.num_out, d_dec)) return rc;
-    IHE_HIP(hipMemcpyAsync(R.r.out + (size_t)m.num_out * col0, d_dec, (size_t)m.num_out * ns * 4,
+    IHE_HIP(hipMemcpyAsync(r.out + (size_t)m.num_out * col0, d_dec, (size_t)m.num_out * ns * 4,
                            hipMemcpyDeviceToHost, st));
     IHE_HIP(hipEventRecord(s.ev, st));
-    s.busy = true;
-    s.req = id;
-    (void)R;
     return IHE_OK;
   }
 
+  // The chunk on slot `s` has ended (mutex held).
   void finish_chunk(Slot &s, int rc, const std::string &err) {
     arena.release(s.off, s.bytes);
-    if (s.key) ihe_key_destroy(s.key);
-    s.key = nullptr;
     s.busy = false;
+    s.enqueued = false;
     auto it = reqs.find(s.req);
     if (it == reqs.end()) return;
     Request &R = it->second;
@@ -201,58 +218,98 @@ struct ihe_server {
     }
   }
 
+  // Assigns queued chunks to free slots and arena blocks, in arrival order
+  // (mutex held); the chunks are enqueued by the caller afterwards.
+  bool admit(std::vector<Issue> &todo) {
+    bool progress = false;
+    while (!pending.empty()) {
+      const uint64_t id = pending.front();
+      Request &R = reqs[id];
+      size_t free_slot = slots.size();
+      for (size_t i = 0; i < slots.size(); ++i)
+        if (!slots[i].busy) {
+          free_slot = i;
+          break;
+        }
+      if (free_slot == slots.size()) break;
+      const uint32_t k = R.issued;
+      const uint32_t ns = std::min(kChunkSamples, R.r.num_samples - k * kChunkSamples);
+      const ChunkPlan p = plan_chunk(models[R.r.model], ns);
+      if (p.bytes > arena.size) {
+        fail_request(id, IHE_ERR_NOMEM, "a chunk of this request exceeds the HBM budget");
+        pending.pop_front();
+        progress = true;
+        continue;
+      }
+      size_t off;
+      if (!arena.alloc(p.bytes, off)) break;  // wait for memory, in arrival order
+      Slot &s = slots[free_slot];
+      s.busy = true;
+      s.enqueued = false;
+      s.req = id;
+      s.off = off;
+      s.bytes = p.bytes;
+      s.plan = p;
+      todo.push_back({free_slot, id, R.r, k});
+      if (++R.issued == R.chunks) pending.pop_front();
+      progress = true;
+    }
+    return progress;
+  }
+
   void run() {
-    std::unique_lock<std::mutex> lk(mu);
+    std::vector<Issue> todo;
+    std::vector<std::pair<size_t, hipError_t>> ended;
     for (;;) {
-      bool progress = false;
-      for (Slot &s : slots) {
-        if (!s.busy) continue;
+      // slots whose chunk has ended (event queries without the mutex)
+      ended.clear();
+      for (size_t i = 0; i < slots.size(); ++i) {
+        Slot &s = slots[i];
+        if (!s.busy || !s.enqueued) continue;
         const hipError_t q = hipEventQuery(s.ev);
         if (q == hipErrorNotReady) continue;
-        finish_chunk(s, q == hipSuccess ? IHE_OK : IHE_ERR_HIP,
-                     q == hipSuccess ? "" : std::string("request failed on the device: ") + hipGetErrorString(q));
-        progress = true;
+        if (s.key) ihe_key_destroy(s.key);
+        s.key = nullptr;
+        ended.push_back({i, q});
       }
-      while (!pending.empty()) {
-        const uint64_t id = pending.front();
-        Request &R = reqs[id];
-        Slot *free_slot = nullptr;
-        for (Slot &s : slots)
-          if (!s.busy) {
-            free_slot = &s;
-            break;
-          }
-        if (!free_slot) break;
-        const uint32_t k = R.issued;
-        const uint32_t ns = std::min(kChunkSamples, R.r.num_samples - k * kChunkSamples);
-        const ChunkPlan p = plan_chunk(models[R.r.model], ns);
-        if (p.bytes > arena.size) {
-          fail_request(id, IHE_ERR_NOMEM, "a chunk of this request exceeds the HBM budget");
-          pending.pop_front();
-          progress = true;
-          continue;
-        }
-        size_t off;
-        if (!arena.alloc(p.bytes, off)) break;  // wait for memory, in arrival order
-        free_slot->off = off;
-        free_slot->bytes = p.bytes;
-        if (int rc = issue(*free_slot, id, R, k, p)) {
+      todo.clear();
+      bool busy = false, progress = !ended.empty();
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        for (const auto &e : ended)
+          finish_chunk(slots[e.first], e.second == hipSuccess ? IHE_OK : IHE_ERR_HIP,
+                       e.second == hipSuccess
+                           ? ""
+                           : std::string("request failed on the device: ") + hipGetErrorString(e.second));
+        progress |= admit(todo);
+        for (const Slot &s : slots) busy |= s.busy;
+        if (stop && !busy && pending.empty()) break;
+        if (!progress)
+          cv_work.wait_for(lk, std::chrono::microseconds(busy ? 20 : 2000));
+      }
+      // enqueue the admitted chunks with the mutex released
+      for (const Issue &t : todo) {
+        Slot &s = slots[t.slot];
+        const int rc = issue(s, t.r, models_snapshot(t.r.model), t.k);
+        if (rc) {
           // nothing of this chunk reached the device queue after the failure
-          (void)hipStreamSynchronize(free_slot->ctx->stream);
-          free_slot->req = id;
-          R.issued++;
-          finish_chunk(*free_slot, rc, ihe_last_error());
+          const std::string err = ihe_last_error();
+          (void)hipStreamSynchronize(s.ctx->stream);
+          if (s.key) ihe_key_destroy(s.key);
+          s.key = nullptr;
+          std::lock_guard<std::mutex> lk(mu);
+          finish_chunk(s, rc, err);
         } else {
-          R.issued++;
+          s.enqueued = true;
         }
-        if (R.issued == R.chunks) pending.pop_front();
-        progress = true;
       }
-      bool busy = false;
-      for (const Slot &s : slots) busy |= s.busy;
-      if (stop && !busy && pending.empty()) break;
-      if (!progress) cv_work.wait_for(lk, std::chrono::microseconds(busy ? 20 : 2000));
     }
+  }
+
+  // A model record, read under the mutex (add_model may grow the table).
+  ModelRec models_snapshot(uint32_t m) {
+    std::lock_guard<std::mutex> lk(mu);
+    return models[m];
   }
 };
 
@@ -302,6 +359,7 @@ int ihe_server_add_model(ihe_server *s, const ihe_band *band, uint32_t num_tags,
     return fail(IHE_ERR_RANGE, "the model references tags beyond num_tags");
   std::lock_guard<std::mutex> lk(s->mu);
   s->models.push_back({band, num_tags, band->num_out});
+  band->users++;
   *model = (uint32_t)s->models.size() - 1;
   return IHE_OK;
 }
@@ -319,7 +377,10 @@ int ihe_server_submit(ihe_server *s, const ihe_request *req, uint64_t *ticket) {
   Request &R = s->reqs[id];
   R.r = *req;
   R.chunks = (req->num_samples + kChunkSamples - 1) / kChunkSamples;
-  R.t_submit = Clock::now();
+  // steady_clock is CLOCK_MONOTONIC, the clock arrival_ns is read from
+  R.t_arrive = req->arrival_ns
+                   ? Clock::time_point(std::chrono::nanoseconds((int64_t)req->arrival_ns))
+                   : Clock::now();
   s->pending.push_back(id);
   *ticket = id;
   s->cv_work.notify_one();
@@ -331,7 +392,7 @@ static int collect(ihe_server *s, std::unordered_map<uint64_t, Request>::iterato
   const Request R = it->second;
   s->reqs.erase(it);
   if (latency_ms)
-    *latency_ms = std::chrono::duration<double, std::milli>(R.t_done - R.t_submit).count();
+    *latency_ms = std::chrono::duration<double, std::milli>(R.t_done - R.t_arrive).count();
   if (R.rc) return fail(R.rc, R.err);
   return IHE_OK;
 }
@@ -382,6 +443,7 @@ int ihe_server_destroy(ihe_server *s) {
     if (sl.ctx) ihe_ctx_destroy(sl.ctx);
   }
   (void)ihe_free(s->ctx, s->arena_base);
+  for (const ModelRec &m : s->models) m.band->users--;
   delete s;
   return IHE_OK;
 }

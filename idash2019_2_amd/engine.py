@@ -53,6 +53,26 @@ class DeviceArray:
             check(lib().ihe_memcpy_d2h(self.engine.ctx, out.ctypes.data, self.ptr, self.nbytes))
         return out
 
+    def take(self, rows) -> np.ndarray:
+        """Rows of the leading axis (any order, repeats allowed) copied to the host,
+        one copy per run of consecutive rows: a few rows of a multi-GB array."""
+        rows = np.asarray(rows, np.int64).ravel()
+        lead = self.shape[0] if self.shape else 0
+        if rows.size and (rows.min() < 0 or rows.max() >= lead):
+            raise IndexError("row index out of range")
+        row_bytes = self.nbytes // lead if lead else 0
+        uniq = np.unique(rows)
+        got = np.empty((uniq.size,) + self.shape[1:], self.dtype)
+        i = 0
+        while i < uniq.size:
+            j = i + 1
+            while j < uniq.size and uniq[j] == uniq[j - 1] + 1:
+                j += 1
+            check(lib().ihe_memcpy_d2h(self.engine.ctx, got[i:j].ctypes.data,
+                                       self.ptr + int(uniq[i]) * row_bytes, (j - i) * row_bytes))
+            i = j
+        return got[np.searchsorted(uniq, rows)]
+
     def fill_zero(self) -> None:
         if self.nbytes:
             check(lib().ihe_memset(self.engine.ctx, self.ptr, 0, self.nbytes))
@@ -90,13 +110,35 @@ class Band:
             self.handle = 0
 
 
+def check_cloud_buffers(band: "Band", num_in: int, d_in, d_in_var, d_out, d_out_var,
+                        what: str = "") -> None:
+    """The sizes the cloud kernels address: num_in input ciphertexts and
+    variances, band.num_out output ciphertexts and variances (ValueError)."""
+    need = ((d_in, num_in * CT_WORDS * 4, "input ciphertext"),
+            (d_in_var, num_in * 8, "input variance"),
+            (d_out, band.num_out * CT_WORDS * 4, "output ciphertext"),
+            (d_out_var, band.num_out * 8, "output variance"))
+    for buf, size, name in need:
+        if buf is not None and buf.nbytes < size:
+            raise ValueError(f"{what}{name} buffer too small ({buf.nbytes} < {size} bytes)")
+    if d_out is None or (num_in > 0 and d_in is None):
+        raise ValueError(f"{what}null ciphertext buffer")
+    if d_out_var is not None and num_in > 0 and d_in_var is None:
+        raise ValueError(f"{what}output variances need input variances")
+
+
 class Batch:
     """Many cohorts' cloud steps as one launch per kernel family (ihe_batch_*)."""
 
     def __init__(self, engine: "Engine", cohorts):
         self.engine = engine
+        for i, c in enumerate(cohorts):
+            band, ns, num_in, d_in, d_in_var, d_out, d_out_var = c
+            layout_for(ns)
+            check_cloud_buffers(band, num_in, d_in, d_in_var, d_out, d_out_var, f"cohort {i}: ")
         arr = (_abi.Cohort * max(1, len(cohorts)))()
         self._keep = []
+        self.handle = 0
         for i, c in enumerate(cohorts):
             band, ns, num_in, d_in, d_in_var, d_out, d_out_var = c
             _, R, RS = layout_for(ns)
@@ -161,11 +203,33 @@ class Server:
         check(lib().ihe_server_create(engine.ctx, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value
         self._live = {}
+        self._models = {}   # model id -> (tags, outputs)
 
     def add_model(self, band: Band, num_tags: int) -> int:
         m = ctypes.c_uint32()
         check(lib().ihe_server_add_model(self.handle, band.handle, num_tags, ctypes.byref(m)))
+        self._models[m.value] = (int(num_tags), int(band.num_out))
         return m.value
+
+    def check_request(self, model: int, geno: np.ndarray, key: np.ndarray, out: np.ndarray,
+                      ns: int) -> None:
+        """The buffer sizes a request's copies use (ValueError), checked before
+        anything reaches the library."""
+        if model not in self._models:
+            raise ValueError(f"unknown model {model}")
+        tags, outputs = self._models[model]
+        if ns < 1:
+            raise ValueError("a request needs at least one sample")
+        if geno.dtype != np.int8 or not geno.flags.c_contiguous:
+            raise ValueError("genotypes must be C-contiguous int8")
+        if out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError("scores must be C-contiguous float32")
+        if geno.size != tags * ns:
+            raise ValueError(f"genotypes hold {geno.size} values, the model needs {tags} x {ns}")
+        if out.size != outputs * ns:
+            raise ValueError(f"scores hold {out.size} values, the model writes {outputs} x {ns}")
+        if key.shape != (N,):
+            raise ValueError("key must have N = 1024 coefficients")
 
     @staticmethod
     def to_chunks(a: np.ndarray) -> np.ndarray:
@@ -186,18 +250,18 @@ class Server:
         return out
 
     def submit(self, model: int, geno: np.ndarray, key: np.ndarray, seed: int,
-               out: np.ndarray, num_samples: int | None = None) -> int:
+               out: np.ndarray, num_samples: int | None = None,
+               arrival_ns: int = 0) -> int:
         """geno int8 and out float32 in the chunk-major request layout (for NS <= 1024
         simply [num_tags][NS] and [num_outputs][NS]; see to_chunks), key [N] int32;
-        all three must stay alive (and unmoved) until the request completes."""
-        ns = num_samples if num_samples is not None else geno.shape[-1]
-        if geno.dtype != np.int8 or not geno.flags.c_contiguous:
-            raise ValueError("genotypes must be C-contiguous int8")
-        if out.dtype != np.float32 or not out.flags.c_contiguous:
-            raise ValueError("scores must be C-contiguous float32")
+        all three must stay alive (and unmoved) until the request completes.
+        arrival_ns: the request's scheduled arrival on time.monotonic_ns()'s clock
+        (its latency is counted from there); 0 counts from the submit."""
+        ns = int(num_samples if num_samples is not None else geno.shape[-1])
         key = np.ascontiguousarray(key, np.int32)
+        self.check_request(model, geno, key, out, ns)
         r = _abi.Request(model, ns, geno.ctypes.data, key.ctypes.data,
-                         seed & 0xFFFFFFFFFFFFFFFF, out.ctypes.data)
+                         seed & 0xFFFFFFFFFFFFFFFF, out.ctypes.data, int(arrival_ns))
         t = ctypes.c_uint64()
         check(lib().ihe_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)))
         self._live[t.value] = (geno, key, out)
@@ -309,6 +373,10 @@ class Engine:
     def timing_reset(self) -> None:
         check(lib().ihe_timing_reset(self.ctx))
 
+    def timing_region(self, begin: bool) -> None:
+        """Start or end event of a region of this context's stream (TIMER_REGION)."""
+        check(lib().ihe_timing_region(self.ctx, int(begin)))
+
     def timing_read(self, timer: int) -> tuple[float, int]:
         ms = ctypes.c_double()
         cnt = ctypes.c_uint32()
@@ -364,10 +432,7 @@ class Engine:
     def cloud(self, band: Band, d_in: DeviceArray, d_in_var: DeviceArray | None,
               d_out: DeviceArray, d_out_var: DeviceArray | None) -> None:
         """cloud_compute_score (eval/idash.cpp:763-848) on device buffers (async)."""
-        if d_out.nbytes < band.num_out * CT_WORDS * 4:
-            raise ValueError("output buffer too small")
-        if d_in.nbytes < band.num_in * CT_WORDS * 4:
-            raise ValueError("input buffer too small")
+        check_cloud_buffers(band, band.num_in, d_in, d_in_var, d_out, d_out_var)
         check(lib().ihe_cloud_score(self.ctx, band.handle, d_in.ptr,
                                     d_in_var.ptr if d_in_var is not None else None, d_out.ptr,
                                     d_out_var.ptr if d_out_var is not None else None))
@@ -378,8 +443,7 @@ class Engine:
         """The band's model on a cohort with another sample count (MFMA bands)."""
         ns, R, RS = layout_for(num_samples)
         lay = _abi.Layout(ns, R, RS)
-        if d_out.nbytes < band.num_out * CT_WORDS * 4 or d_in.nbytes < num_in * CT_WORDS * 4:
-            raise ValueError("buffer too small")
+        check_cloud_buffers(band, num_in, d_in, d_in_var, d_out, d_out_var)
         check(lib().ihe_cloud_score_layout(self.ctx, band.handle, ctypes.byref(lay), num_in,
                                            d_in.ptr, d_in_var.ptr if d_in_var is not None else None,
                                            d_out.ptr,

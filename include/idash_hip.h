@@ -34,7 +34,7 @@ enum {
   IHE_ERR_RANGE = 3,       /* model references a ciphertext that is not present */
   IHE_ERR_NOMEM = 4,       /* device or host allocation failed / over budget */
   IHE_ERR_UNSUPPORTED = 5, /* shape the kernels do not handle */
-  IHE_ERR_BUSY = 6         /* serving queue full */
+  IHE_ERR_BUSY = 6         /* serving queue full; band still held by a batch or server */
 };
 
 #define IHE_N 1024u                     /* TRLWE dimension, eval/idash.h:49 */
@@ -133,10 +133,16 @@ enum {
   IHE_TIMER_DECRYPT = 3,
   IHE_TIMER_BATCH = 4,
   IHE_TIMER_STREAM = 5,    /* ihe_stream_probe */
-  IHE_TIMER_COUNT = 6
+  IHE_TIMER_REGION = 6,    /* caller-marked regions (ihe_timing_region) */
+  IHE_TIMER_COUNT = 7
 };
 /* When enabled, each launch is bracketed by HIP events on the launch stream. */
 int ihe_timing_enable(ihe_ctx *ctx, int on);
+/* Records the start (begin != 0) or the end (begin == 0) of a region of the
+ * context stream's work under IHE_TIMER_REGION, whether or not per-launch
+ * timing is enabled: a timed loop bracketed by two events, with nothing
+ * recorded between its launches. */
+int ihe_timing_region(ihe_ctx *ctx, int begin);
 int ihe_timing_reset(ihe_ctx *ctx);
 /* Synchronizes, then returns the summed duration and launch count since reset. */
 int ihe_timing_read(ihe_ctx *ctx, int timer, double *total_ms, uint32_t *count);
@@ -163,6 +169,7 @@ int ihe_band_create_tuned(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_i
                           const ihe_model_csr *model, const int32_t *d_in, int32_t *d_out,
                           uint32_t reps, ihe_band **out, ihe_tune_entry *results,
                           uint32_t *num_results);
+/* IHE_ERR_BUSY while a batch or server holds the band. */
 int ihe_band_destroy(ihe_band *band);
 int ihe_band_get_info(const ihe_band *band, ihe_band_info *info);
 /* Runs the band compiler on the host only (no device, no upload) and reports
@@ -207,6 +214,9 @@ typedef struct {
 typedef struct {
   uint32_t num_cohorts, num_launches, num_tiles, captured;
 } ihe_batch_info;
+/* The batch keeps device pointers to the cohorts' buffers and bands: both must
+ * outlive it (ihe_band_destroy fails with IHE_ERR_BUSY while a batch holds the
+ * band; buffers are the caller's to keep). */
 int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohorts,
                      ihe_batch **out);
 int ihe_batch_capture(ihe_batch *batch);
@@ -240,6 +250,7 @@ typedef struct {
   const int32_t *key;   /* host [N] binary key */
   uint64_t seed;        /* chunk k encrypts with seed + k */
   float *out;           /* host, chunk-major [num_outputs][ns_k] blocks of decrypted scores */
+  uint64_t arrival_ns;  /* scheduled arrival, CLOCK_MONOTONIC nanoseconds; 0: the submit time */
 } ihe_request;
 typedef struct {
   uint64_t completed;
@@ -247,13 +258,16 @@ typedef struct {
   uint64_t arena_bytes, arena_used, arena_peak;
 } ihe_server_stats;
 int ihe_server_create(ihe_ctx *ctx, const ihe_server_config *cfg, ihe_server **out);
-/* band: an MFMA band over the big indices of num_tags tag SNPs. */
+/* band: an MFMA band over the big indices of num_tags tag SNPs. The band must
+ * outlive the server: ihe_band_destroy fails with IHE_ERR_BUSY until the
+ * server is destroyed. */
 int ihe_server_add_model(ihe_server *server, const ihe_band *band, uint32_t num_tags,
                          uint32_t *model);
 /* The request's host buffers must stay valid until it completes. */
 int ihe_server_submit(ihe_server *server, const ihe_request *req, uint64_t *ticket);
-/* Completed requests report their latency (submit to scores on the host) and
- * are forgotten; one waiter per ticket. */
+/* Completed requests report their latency (arrival, or submit when the
+ * request gives none, to scores on the host) and are forgotten; one waiter per
+ * ticket. */
 int ihe_server_poll(ihe_server *server, uint64_t ticket, int *done, double *latency_ms);
 int ihe_server_wait(ihe_server *server, uint64_t ticket, double *latency_ms);
 int ihe_server_get_stats(ihe_server *server, ihe_server_stats *stats);

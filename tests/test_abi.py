@@ -184,3 +184,45 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+class _Buf:
+    """Stands in for a DeviceArray: the checks read nbytes only."""
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        self.ptr = 1
+
+
+def test_batch_checks_buffer_sizes_before_the_library():
+    from idash2019_2_amd.engine import Band, Batch
+    band = Band(engine=None, handle=0, num_in=10, num_out=20, layout=(1024, 1, 1024))
+    ok = (band, 1024, 10, _Buf(10 * 8192), _Buf(80), _Buf(20 * 8192), _Buf(160))
+    bad_cases = [
+        (band, 1024, 10, _Buf(10 * 8192 - 4), _Buf(80), _Buf(20 * 8192), _Buf(160)),
+        (band, 1024, 10, _Buf(10 * 8192), _Buf(72), _Buf(20 * 8192), _Buf(160)),
+        (band, 1024, 10, _Buf(10 * 8192), _Buf(80), _Buf(19 * 8192), _Buf(160)),
+        (band, 1024, 10, _Buf(10 * 8192), _Buf(80), _Buf(20 * 8192), _Buf(152)),
+        (band, 1024, 10, _Buf(10 * 8192), None, _Buf(20 * 8192), _Buf(160)),
+        (band, 0, 10, _Buf(10 * 8192), _Buf(80), _Buf(20 * 8192), _Buf(160)),
+    ]
+    for bad in bad_cases:
+        with pytest.raises(ValueError):
+            Batch(None, [ok, bad])          # engine None: any library call would fail first
+
+
+def test_server_checks_request_sizes_before_the_library():
+    from idash2019_2_amd.engine import Server
+    srv = Server.__new__(Server)            # no device: only the host-side checks run
+    srv.handle, srv._live, srv._models = 0, {}, {0: (50, 360)}
+    geno = np.zeros(50 * 7, np.int8)
+    out = np.zeros(360 * 7, np.float32)
+    key = np.zeros(1024, np.int32)
+    srv.check_request(0, geno, key, out, 7)
+    for args in [(1, geno, key, out, 7), (0, geno[:-1], key, out, 7), (0, geno, key, out[:-1], 7),
+                 (0, geno, key[:-1], out, 7), (0, geno, key, out, 0),
+                 (0, geno.astype(np.int16), key, out, 7), (0, geno, key, out.astype(np.float64), 7)]:
+        with pytest.raises(ValueError):
+            srv.check_request(*args)
+    with pytest.raises(ValueError):
+        srv.submit(0, geno, key, 1, out[:-7], num_samples=7)

@@ -135,9 +135,13 @@ def test_batch_without_variances_and_errors(engine, setup):
     assert e.value.code == _abi.IHE_ERR_RANGE
     c, csr = models[(0, 5)]
     valu = engine.band(1024, c.num_in_cts, *csr, variant=_abi.VARIANT_VALU)
+    d_in_1024 = engine.empty((c.num_in_cts, 2, 1024), np.int32)
     with pytest.raises(IheError) as e:
-        engine.batch([(valu, 1024, c.num_in_cts, d_in, d_var, d_out, d_ov)])
+        engine.batch([(valu, 1024, c.num_in_cts, d_in_1024, None, d_out, None)])
     assert e.value.code == _abi.IHE_ERR_UNSUPPORTED
+    with pytest.raises(ValueError):                 # buffers below the sizes the step addresses
+        engine.batch([(band, ns, num_in, d_in, d_var, d_out, d_ov),
+                       (valu, 1024, c.num_in_cts, d_in, None, d_out, None)])
     valu.free()
 
 
@@ -158,4 +162,21 @@ def test_tuned_band_matches_default(engine, n):
     engine.cloud(band, d_in, d_var, d_out, d_ov)
     want, wv = oracle.cloud(ct, var, rp, ib, w, 1024, 1, 1024)
     assert (d_out.numpy() == want).all() and (d_ov.numpy() == wv).all()
+    band.free()
+
+
+def test_a_band_outlives_its_batch(engine, setup):
+    """ihe_band_destroy refuses a band a live batch holds (IHE_ERR_BUSY), and
+    frees it once the batch is gone."""
+    models, _ = setup
+    c, csr = models[(0, 5)]
+    band = engine.band(1024, c.num_in_cts, *csr)
+    d_in = engine.empty((c.num_in_cts, 2, 1024), np.int32)
+    d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
+    batch = engine.batch([(band, 1024, c.num_in_cts, d_in, None, d_out, None),
+                          (band, 512, c.num_in_cts, d_in, None, d_out, None)])
+    with pytest.raises(IheError) as e:
+        band.free()
+    assert e.value.code == _abi.IHE_ERR_BUSY
+    batch.free()
     band.free()

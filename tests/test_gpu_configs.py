@@ -1,0 +1,339 @@
+"""The five workloads at their stated sizes, on the GPU, against the oracle.
+
+  small    500 tags / 2,000 targets / 128 samples (R = 8), n = 5: every output
+           of encrypt, cloud and decrypt, through the engine and through the
+           idash-compatible executables
+  wide     the full 16,184 / 80,882 layout at 1,024 samples with n = 50: 600
+           random rows plus the first and last, every variance by closed form
+  batched  64 cohorts on the full layout, NS ~ U[16, 1024], 3 model sets,
+           n in {5, 10, 20, 50}, one batch (eager and as a graph): 32 sampled
+           rows of every cohort against the oracle at its own R, every variance
+           by closed form, every output equal to its per-cohort launch
+  served   requests of NS log-uniform in [1, 4096] (chunks of <= 1024 above
+           1024) on 3 resident full-layout model sets: 32 sampled rows of every
+           chunk against oracle encrypt -> cloud -> decrypt, and every request
+           the same alone and in the mix
+
+(The full n = 5 workload is checked in test_gpu_cloud.py.) The reference loop
+being restated is cloud_compute_score (eval/idash.cpp:763-848), with encrypt
+(:615-679) and decrypt (:681-761) around it.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from idash2019_2_amd import Server, _abi, formats, synth
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "idash2019_2_amd", "host")
+CONST = _abi.IHE_CONSTANT_BIDX
+VAR0 = 2.0 ** -50          # variance of a fresh encryption (alpha^2)
+
+
+def sub_csr(rp, ib, w, rows):
+    """The model restricted to `rows` (CSR over the listed rows, in order)."""
+    rows = np.asarray(rows, np.int64)
+    lens = (rp[rows + 1] - rp[rows]).astype(np.int64)
+    srp = np.r_[0, np.cumsum(lens)].astype(np.uint32)
+    sib = np.concatenate([ib[rp[r]:rp[r + 1]] for r in rows])
+    sw = np.concatenate([w[rp[r]:rp[r + 1]] for r in rows])
+    return srp, sib, sw
+
+
+def needed_cts(ib, R):
+    """FeatIndex of every non-constant big index."""
+    ib = np.asarray(ib, np.uint32)
+    return np.unique(ib[ib != CONST] // R).astype(np.int64)
+
+
+def window_variances(c, R):
+    """Closed form of every output variance of a windowed model whose inputs all
+    carry alpha^2: 2^-50 * sum of w*w over the window columns c with c % R == 0
+    (eval/idash.cpp:817, 825: only region 0 reaches the output). Integer sums
+    below 2^53, so the double result is exact in any summation order."""
+    n3 = 3 * c.neighbours
+    cols = 3 * np.repeat(c.lo.astype(np.int64), 3)[:, None] + np.arange(n3)[None, :]
+    w2 = c.weights.astype(np.int64) ** 2
+    return (w2 * (cols % R == 0)).sum(axis=1).astype(np.float64) * VAR0
+
+
+def rows_against_oracle(c, rp, ib, w, rows, d_in, d_out, layout):
+    """Oracle cloud of `rows` on the input ciphertexts those rows read (fetched
+    from the device into an otherwise empty array) against the device rows."""
+    ns, R, RS = layout
+    srp, sib, sw = sub_csr(rp, ib, w, rows)
+    need = needed_cts(sib, R)
+    ct = np.zeros((d_in.shape[0], 2, 1024), np.int32)
+    ct[need] = d_in.take(need)
+    var = np.full(d_in.shape[0], VAR0)
+    ref, _ = oracle.cloud(ct, var, srp, sib, sw, ns, R, RS)
+    got = d_out.take(rows)
+    bad = np.nonzero((got != ref).any(axis=(1, 2)))[0]
+    assert bad.size == 0, f"rows {np.asarray(rows)[bad[:8]]} differ (NS={ns})"
+
+
+def sample_rows(rng, num_out, k):
+    return np.unique(np.r_[0, num_out - 1, rng.choice(num_out, k, replace=False)])
+
+
+# ---------------------------------------------------------------- small
+
+@pytest.mark.parametrize("kernel", ["auto", "staged", "direct"])
+def test_small_workload_every_output(engine, kernel, monkeypatch):
+    """Encrypt, cloud and decrypt of the whole small workload (188 input and
+    6,000 output ciphertexts, NS = 128, R = 8) equal the oracle word for word."""
+    monkeypatch.delenv("IHE_CLOUD_KERNEL", raising=False)
+    if kernel != "auto":
+        monkeypatch.setenv("IHE_CLOUD_KERNEL", kernel)
+    c = synth.workload("small")
+    ns, R, RS = c.layout
+    assert (ns, R, RS, c.num_in_cts, c.num_outputs) == (128, 8, 128, 188, 6000)
+    key = engine.key(c.key)
+    d_geno = engine.to_device(c.geno)
+    d_ct = engine.empty((c.num_in_cts, 2, 1024), np.int32)
+    d_var = engine.empty((c.num_in_cts,), np.float64)
+    engine.encrypt(key, ns, d_geno, 31337, d_ct, d_var)
+    cts, var = oracle.encrypt(c.geno, c.key, ns, R, RS, seed=31337)
+    assert (d_ct.numpy() == cts).all() and (d_var.numpy() == var).all()
+    rp, ib, w = c.model_csr()
+    band = engine.band(ns, c.num_in_cts, rp, ib, w)
+    want_variant = _abi.VARIANT_MFMA if kernel == "auto" else _abi.VARIANT_VALU
+    assert band.info()["variant"] == want_variant
+    d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
+    d_ov = engine.empty((c.num_outputs,), np.float64)
+    engine.cloud(band, d_ct, d_var, d_out, d_ov)
+    ref, rv = oracle.cloud(cts, var, rp, ib, w, ns, R, RS)
+    assert (d_out.numpy() == ref).all() and (d_ov.numpy() == rv).all()
+    d_dec = engine.empty((c.num_outputs, ns), np.float32)
+    engine.decrypt(key, ns, d_out, c.num_outputs, d_dec)
+    assert (d_dec.numpy() == oracle.decrypt(ref, c.key, ns)).all()
+    band.free()
+    key.free()
+
+
+def test_small_workload_through_the_executables(tmp_path):
+    """keygen -> encrypt -> cloud -> decrypt (eval/keygen.cpp, encrypt.cpp,
+    cloud.cpp, decrypt.cpp) on the small workload's files: every prediction
+    record bit-exact against the oracle's cloud step on the records encrypt
+    wrote, every result.csv line against the oracle's decryption."""
+    c = synth.workload("small")
+    synth.write_reference_files(c, str(tmp_path))
+    env = dict(os.environ, IHE_SEED="12")
+    for exe, args in (("keygen", ["targets.txt", "tags.txt", "1"]), ("encrypt", ["tags.txt"]),
+                      ("cloud", ["model"]), ("decrypt", [])):
+        r = subprocess.run([os.path.join(HOST, exe), *args], cwd=tmp_path, capture_output=True,
+                           text=True, timeout=300, env=env)
+        assert r.returncode == 0, f"{exe}: {r.stdout}{r.stderr}"
+    p, key, _ = formats.read_key(str(tmp_path / "keys.bin"))
+    NS, R, RS = p["NUM_SAMPLES"], p["NUM_REGIONS"], p["REGION_SIZE"]
+    assert (NS, R, RS) == c.layout
+    idx, ct, var = formats.read_records(str(tmp_path / "encrypted_data.bin"), c.num_in_cts)
+    assert sorted(idx.tolist()) == list(range(c.num_in_cts))
+    pidx, pred, pvar = formats.read_records(str(tmp_path / "encrypted_prediction.bin"),
+                                            c.num_outputs)
+    assert sorted(pidx.tolist()) == list(range(c.num_outputs))
+    rp, ib, w = c.model_csr()
+    ref, rv = oracle.cloud(ct, var, rp, ib, w, NS, R, RS)
+    assert (pred == ref).all() and (pvar == rv).all()
+    dec = oracle.decrypt(ref, key, NS)
+    lines = open(tmp_path / "result.csv").read().split("\n")
+    assert lines[0] == "Subject ID,target SNP,0,1,2" and lines[-1] == ""
+    body = lines[1:-1]
+    assert len(body) == NS * c.num_targets
+    names = [str(x) for x in c.target_pos.tolist()]
+    for k, line in enumerate(body):
+        s, t = divmod(k, c.num_targets)
+        want = "%d,%s,%g,%g,%g" % (s, names[t], dec[3 * t, s], dec[3 * t + 1, s], dec[3 * t + 2, s])
+        assert line == want, (k, line, want)
+
+
+# ---------------------------------------------------------------- wide
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("kernel", ["auto", "staged"])
+def test_wide_workload_full_layout(engine, kernel, monkeypatch):
+    """n = 50 on the full layout: 150 one-hot columns per output, 242,646
+    outputs from 48,552 device-encrypted inputs; the MFMA kernel (the default
+    for wide windows) and the LDS-staged VALU kernel."""
+    monkeypatch.delenv("IHE_CLOUD_KERNEL", raising=False)
+    if kernel != "auto":
+        monkeypatch.setenv("IHE_CLOUD_KERNEL", kernel)
+    c = synth.workload("wide")
+    ns, R, RS = c.layout
+    assert (c.num_in_cts, c.num_outputs, c.neighbours) == (48552, 242646, 50)
+    key = engine.key(c.key)
+    d_geno = engine.to_device(c.geno)
+    d_ct = engine.empty((c.num_in_cts, 2, 1024), np.int32)
+    d_var = engine.empty((c.num_in_cts,), np.float64)
+    engine.encrypt(key, ns, d_geno, 5150, d_ct, d_var)
+    d_geno.free()
+    rp, ib, w = c.model_csr()
+    band = engine.band(ns, c.num_in_cts, rp, ib, w)
+    info = band.info()
+    assert info["max_window"] == 150
+    if kernel == "auto":
+        assert info["variant"] == _abi.VARIANT_MFMA and info["tile_rows"] == 128
+    d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
+    d_ov = engine.empty((c.num_outputs,), np.float64)
+    engine.cloud(band, d_ct, d_var, d_out, d_ov)
+    rng = np.random.default_rng(50)
+    rows = sample_rows(rng, c.num_outputs, 600)
+    rows_against_oracle(c, rp, ib, w, rows, d_ct, d_out, c.layout)
+    assert (d_ov.numpy() == window_variances(c, R)).all()
+    for d in (band, key):
+        d.free()
+    for d in (d_ct, d_var, d_out, d_ov):
+        d.free()
+
+
+# ---------------------------------------------------------------- batched
+
+@pytest.mark.timeout(900)
+def test_batched_workload_full_layout(engine):
+    """64 cohorts drawn as the batched workload is: NS ~ U[16, 1024] (R from 1
+    to 64), one of 3 model sets, n in {5, 10, 20, 50}, all on the 16,184-tag /
+    80,882-target layout; inputs encrypted on the device with a key per cohort."""
+    base = synth.workload("full")
+    rng = np.random.default_rng(4)
+    models = synth.model_sets(base, (5, 10, 20, 50), 3, 4)
+    specs = [(int(rng.integers(16, 1025)), int(rng.integers(0, 3)),
+              int(rng.choice([5, 10, 20, 50]))) for _ in range(64)]
+    csrs = {k: models[k].model_csr() for k in sorted({(s, n) for _, s, n in specs})}
+    bands = {k: engine.band(1024, base.num_in_cts, *csr) for k, csr in csrs.items()}
+    cohorts, keys = [], []
+    for i, (ns, s, n) in enumerate(specs):
+        R = 1024 // ns
+        num_in = (3 * base.num_tags + R - 1) // R
+        cols = rng.choice(1024, ns, replace=False)
+        key = engine.key(rng.integers(0, 2, 1024).astype(np.int32))
+        d_g = engine.to_device(np.ascontiguousarray(base.geno[:, cols]))
+        d_in = engine.empty((num_in, 2, 1024), np.int32)
+        d_iv = engine.empty((num_in,), np.float64)
+        engine.encrypt(key, ns, d_g, 4000 + i, d_in, d_iv)
+        d_g.free()
+        key.free()
+        d_out = engine.empty((base.num_outputs, 2, 1024), np.int32)
+        d_ov = engine.empty((base.num_outputs,), np.float64)
+        cohorts.append((bands[(s, n)], ns, num_in, d_in, d_iv, d_out, d_ov))
+    packings = {1024 // ns for ns, _, _ in specs}
+    assert len(packings) >= 8 and 1 in packings and max(packings) >= 16
+    batch = engine.batch(cohorts)
+    batch.run()
+    engine.sync()
+    sums = [(engine.checksum(co[5]), engine.checksum(co[6])) for co in cohorts]
+    for i, ((ns, s, n), co) in enumerate(zip(specs, cohorts)):
+        layout = (ns, 1024 // ns, 1024 // (1024 // ns))
+        rp, ib, w = csrs[(s, n)]
+        rows = sample_rows(np.random.default_rng(i), base.num_outputs, 32)
+        rows_against_oracle(models[(s, n)], rp, ib, w, rows, co[3], co[5], layout)
+        assert (co[6].numpy() == window_variances(models[(s, n)], layout[1])).all(), i
+    # captured as a graph: every replay equals the eager batch
+    batch.capture()
+    for _ in range(2):
+        for co in cohorts:
+            co[5].fill_zero()
+            co[6].fill_zero()
+        engine.sync()
+        batch.run()
+        engine.sync()
+        assert [(engine.checksum(co[5]), engine.checksum(co[6])) for co in cohorts] == sums
+    batch.free()
+    # and every cohort equals its own per-cohort launch
+    for co, want in zip(cohorts, sums):
+        band, ns, num_in, d_in, d_iv, d_out, d_ov = co
+        d_out.fill_zero()
+        engine.cloud_layout(band, ns, num_in, d_in, d_iv, d_out, d_ov)
+        engine.sync()
+        assert (engine.checksum(d_out), engine.checksum(d_ov)) == want
+        for d in co[3:]:
+            d.free()
+    for b in bands.values():
+        b.free()
+
+
+# ---------------------------------------------------------------- served
+
+def served_requests(base, count, seed):
+    """NS log-uniform in [1, 4096]; at least two requests above 1024 samples."""
+    rng = np.random.default_rng(seed)
+    while True:
+        ns = np.exp(rng.uniform(0, np.log(4096), count)).astype(int).clip(1, 4096)
+        if (ns > 1024).sum() >= 2 and ns.sum() <= 14000:
+            break
+    reqs = []
+    for i, n in enumerate(ns.tolist()):
+        cols = rng.integers(0, base.num_samples, n)
+        geno = np.ascontiguousarray(base.geno[:, cols])
+        key = rng.integers(0, 2, 1024).astype(np.int32)
+        reqs.append((int(rng.integers(0, 3)), geno, key, 20_000 + 11 * i))
+    return reqs
+
+
+def digest(a):
+    v = a.reshape(-1).view(np.uint32)
+    return int(v.astype(np.uint64).sum()), int(np.bitwise_xor.reduce(v)), a.size
+
+
+@pytest.mark.timeout(900)
+def test_served_workload_full_layout(engine):
+    base = synth.workload("full")
+    models = synth.model_sets(base, (5,), 3, 5)
+    csrs = {s: models[(s, 5)].model_csr() for s in range(3)}
+    bands = {s: engine.band(1024, base.num_in_cts, *csrs[s]) for s in range(3)}
+    reqs = served_requests(base, 9, 55)
+    outs = [np.empty(base.num_outputs * g.shape[1], np.float32) for _, g, _, _ in reqs]
+
+    def serve(slots, together):
+        srv = engine.server(96 << 30, num_slots=slots)
+        mids = [srv.add_model(bands[s], base.num_tags) for s in range(3)]
+        tickets = []
+        for (s, geno, key, seed), out in zip(reqs, outs):
+            out.fill(np.nan)
+            t = srv.submit(mids[s], Server.to_chunks(geno), key, seed, out,
+                           num_samples=geno.shape[1])
+            if together:
+                tickets.append(t)
+            else:
+                srv.wait(t)
+        for t in tickets:
+            srv.wait(t)
+        st = srv.stats()
+        srv.close()
+        return st
+
+    st = serve(4, True)
+    assert st["completed"] == len(reqs) and st["arena_peak"] <= st["arena_bytes"]
+    mixed = [digest(o) for o in outs]
+    num_out = base.num_outputs
+    for i, ((s, geno, key, seed), out) in enumerate(zip(reqs, outs)):
+        rp, ib, w = csrs[s]
+        ns_total = geno.shape[1]
+        off = 0
+        for k, col0 in enumerate(range(0, ns_total, 1024)):
+            ns = min(1024, ns_total - col0)
+            R = 1024 // ns
+            RS = 1024 // R
+            block = out[off:off + num_out * ns].reshape(num_out, ns)
+            off += num_out * ns
+            rows = sample_rows(np.random.default_rng(100 * i + k), num_out, 32)
+            srp, sib, sw = sub_csr(rp, ib, w, rows)
+            need = needed_cts(sib, R)
+            chunk = np.ascontiguousarray(geno[:, col0:col0 + ns])
+            sub, sv = oracle.encrypt_cts(chunk, key, ns, R, RS, seed + k, need)
+            num_in = (3 * base.num_tags + R - 1) // R
+            ct = np.zeros((num_in, 2, 1024), np.int32)
+            var = np.full(num_in, VAR0)
+            ct[need] = sub
+            ref, _ = oracle.cloud(ct, var, srp, sib, sw, ns, R, RS)
+            want = oracle.decrypt(ref, key, ns)
+            assert np.array_equal(block[rows], want), (i, k, ns)
+    # one request at a time on one slot: the same scores
+    serve(1, False)
+    assert [digest(o) for o in outs] == mixed
+    for b in bands.values():
+        b.free()

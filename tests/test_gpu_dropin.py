@@ -39,9 +39,10 @@ def run_all(exe_dir, cwd):
 @pytest.mark.skipif(not all(os.path.exists(os.path.join(THEIRS, e)) for e, _ in STEPS),
                     reason="reference executables not built (oracle/dropin.mk needs the "
                            "reference sources)")
-@pytest.mark.parametrize("ns", [128, 1024, 100])
+@pytest.mark.parametrize("ns", [128, 1024, 100, "small"])
 def test_reference_mains_write_the_same_files(tmp_path, ns):
-    c = synth.generate(60, 150, ns, 5, seed=ns + 1)
+    # "small": the 500-tag / 2,000-target / 128-sample workload itself
+    c = synth.workload("small") if ns == "small" else synth.generate(60, 150, ns, 5, seed=ns + 1)
     a, b = tmp_path / "theirs", tmp_path / "mine"
     for d in (a, b):
         d.mkdir()

@@ -133,3 +133,32 @@ def test_oversized_request_fails_cleanly(engine, models):
         srv.submit(m, geno, np.full(1024, 2, np.int32), 1, out)   # key not binary
     assert e.value.code == _abi.IHE_ERR_ARG
     srv.close()
+
+
+def test_latency_counts_from_the_scheduled_arrival(engine, models):
+    """A request whose arrival lies 80 ms before its submit reports a latency of
+    at least 80 ms; without one, latency counts from the submit."""
+    import time
+    c, band = models[5]
+    srv = engine.server(4 * GIB, num_slots=2)
+    m = srv.add_model(band, c.num_tags)
+    geno = np.zeros((c.num_tags, 16), np.int8)
+    out = np.zeros((c.num_outputs, 16), np.float32)
+    key = np.zeros(1024, np.int32)
+    t = srv.submit(m, geno, key, 1, out, arrival_ns=time.monotonic_ns() - 80_000_000)
+    assert srv.wait(t) >= 80.0
+    t = srv.submit(m, geno, key, 1, out)
+    assert srv.wait(t) < 80.0
+    srv.close()
+
+
+def test_a_band_outlives_its_server(engine, models):
+    c, band = models[20]
+    b2 = engine.band(1024, c.num_in_cts, *c.model_csr())
+    srv = engine.server(1 * GIB, num_slots=1)
+    srv.add_model(b2, c.num_tags)
+    with pytest.raises(IheError) as e:
+        b2.free()
+    assert e.value.code == _abi.IHE_ERR_BUSY
+    srv.close()
+    b2.free()
