@@ -138,10 +138,10 @@ def test_cloud_passes_the_framing_word_through(tmp_path):
     assert (formats.read_stream_words(os.path.join(d, "encrypted_prediction.bin")) == 0).all()
     path = os.path.join(d, "encrypted_data.bin")
     idx, ct, var = formats.read_records(path)
-    formats.write_records(path, idx, ct, var, stream_word=0x5A17F00D - (1 << 32))
+    formats.write_records(path, idx, ct, var, stream_word=-0x5A17F00D)
     run("cloud", "model", cwd=d)
     words = formats.read_stream_words(os.path.join(d, "encrypted_prediction.bin"))
-    assert words.size == c.num_outputs and (words == 0x5A17F00D - (1 << 32)).all()
+    assert words.size == c.num_outputs and (words == -0x5A17F00D).all()
     pred1 = formats.read_records(os.path.join(d, "encrypted_prediction.bin"))
     assert (pred0[1] == pred1[1]).all() and (pred0[2] == pred1[2]).all()
     run("decrypt", cwd=d)
