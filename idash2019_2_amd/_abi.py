@@ -129,6 +129,7 @@ SIGNATURES = {
     "ihe_key_destroy": (_int, [_vp]),
     "ihe_encrypt": (_int, [_vp, _vp, ctypes.POINTER(Layout), _u32, _vp, _u64, _vp, _vp]),
     "ihe_decrypt": (_int, [_vp, _vp, _u32, _vp, _u32, _vp]),
+    "ihe_decrypt_noalloc": (_int, [_vp, _u32, _vp, _u32, _vp, _vp]),
     "ihe_noise_table": (_int, [_vp, _u32]),
 }
 

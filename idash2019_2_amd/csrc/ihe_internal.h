@@ -77,5 +77,8 @@ void build_noise_lut(const uint64_t *table, uint32_t len, uint32_t *lut);
 // Work counters of the persistent ring kernels: one per XCD, then a count of
 // finished workgroups (the last one clears all of them for the next launch).
 constexpr uint32_t kRingQueueWords = 16;
+// Looks up (once per device, that device current) the device address of the
+// work counters ihe_decrypt_noalloc uses; ihe_ctx_create calls it.
+int noalloc_setup(int device);
 
 }  // namespace ihe

@@ -17,8 +17,17 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "ihe_internal.h"
+
+// Work counters of ihe_decrypt_noalloc: the ring kernel's per-XCD item
+// counters and its count of finished workgroups, in a file-scope device array
+// instead of a context's allocation. Each GPU holds its own copy for the life
+// of the process; host code uses the address hipGetSymbolAddress gives on
+// that device, never this name.
+static __device__ __align__(256) uint32_t scratch[ihe::kRingQueueWords];
 
 namespace ihe {
 
@@ -595,18 +604,78 @@ __global__ __launch_bounds__(256) void k_encrypt_ae(EncryptParams P) {
   }
 }
 
+// Largest decrypt of ihe_decrypt_noalloc: its item count (32-ciphertext groups
+// x column groups of 1024 words) stays far inside the 32-bit counters.
+constexpr uint32_t kNoallocMaxCts = 1u << 26;
+
 // IHE_RING_TILES = 1 or 2 column tiles per wave (default 2); IHE_RING_QUEUE=0
-// gives the workgroups fixed items instead of the context's counters
+// gives the workgroups fixed items instead of the counters
 template <bool ENC>
-void launch_ring(ihe_ctx *ctx, DecryptParams P, const KeyBits &key, uint32_t blocks) {
+void launch_ring_on(hipStream_t stream, DecryptParams P, const KeyBits &key, uint32_t blocks,
+                    uint32_t *queue) {
   static_assert(kRingQueueWords >= 9, "eight XCD counters and a finished count");
   const char *q = std::getenv("IHE_RING_QUEUE");
-  P.queue = (q && std::strcmp(q, "0") == 0) ? nullptr : ctx->d_ring_queue;
+  P.queue = (q && std::strcmp(q, "0") == 0) ? nullptr : queue;
   const char *e = std::getenv("IHE_RING_TILES");
   if (e && std::strcmp(e, "1") == 0)
-    hipLaunchKernelGGL((k_ring_mfma<ENC, 1>), dim3(blocks), dim3(64 * 8), 0, ctx->stream, P, key);
+    hipLaunchKernelGGL((k_ring_mfma<ENC, 1>), dim3(blocks), dim3(64 * 8), 0, stream, P, key);
   else
-    hipLaunchKernelGGL((k_ring_mfma<ENC, 2>), dim3(blocks), dim3(64 * 4), 0, ctx->stream, P, key);
+    hipLaunchKernelGGL((k_ring_mfma<ENC, 2>), dim3(blocks), dim3(64 * 4), 0, stream, P, key);
+}
+
+template <bool ENC>
+void launch_ring(ihe_ctx *ctx, DecryptParams P, const KeyBits &key, uint32_t blocks) {
+  launch_ring_on<ENC>(ctx->stream, P, key, blocks, ctx->d_ring_queue);
+}
+
+// Decrypt launch shape: items of 32 ciphertexts x 32 * kDecCols words, run by
+// persistent workgroups, two per CU, a multiple of the 8 XCDs.
+DecryptParams decrypt_params(const int32_t *d_in, uint32_t num_samples, uint32_t num_cts,
+                             float *d_out, uint32_t num_cus, uint32_t &blocks) {
+  const uint32_t col_groups = (num_samples + kDecCols - 1) / kDecCols;
+  DecryptParams P{d_in, d_out, nullptr, num_cts, num_samples, col_groups,
+                  ((num_cts + 31) / 32) * col_groups, nullptr};
+  blocks = std::min<uint32_t>((num_cus * 2 + 7) / 8 * 8, (P.num_items + 7) / 8 * 8);
+  return P;
+}
+
+__global__ void k_clear_words(uint32_t *p, uint32_t n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// Per device: the address of `scratch` and the CU count, looked up on the
+// first ihe_decrypt_noalloc call on that device (with that device current).
+constexpr int kMaxDevices = 64;
+struct NoallocDevice {
+  uint32_t *scratch = nullptr;
+  uint32_t num_cus = 0;
+};
+std::mutex g_noalloc_mu;
+NoallocDevice g_noalloc[kMaxDevices];
+
+int noalloc_device(int dev, NoallocDevice &out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(IHE_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  NoallocDevice &d = g_noalloc[dev];
+  if (!d.scratch) {
+    hipDeviceProp_t prop;
+    IHE_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      return fail(IHE_ERR_UNSUPPORTED, std::string("kernels are built for gfx950, device is ") +
+                                           prop.gcnArchName);
+    void *p = nullptr;
+    IHE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(IHE_ERR_HIP, "hipGetSymbolAddress gave no address for the work counters");
+    d.num_cus = (uint32_t)prop.multiProcessorCount;
+    d.scratch = static_cast<uint32_t *>(p);
+  }
+  out = d;
+  return IHE_OK;
+}
+
+int noalloc_setup(int device) {
+  NoallocDevice d;
+  return noalloc_device(device, d);
 }
 
 }  // namespace ihe
@@ -703,12 +772,9 @@ int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const in
   ScopedTimer t(ctx, IHE_TIMER_DECRYPT);
   const char *force = std::getenv("IHE_DECRYPT_KERNEL");
   if (!(force && std::strcmp(force, "valu") == 0)) {
-    const uint32_t col_groups = (num_samples + kDecCols - 1) / kDecCols;
-    DecryptParams P{d_in, d_out, nullptr, num_cts, num_samples, col_groups,
-                    ((num_cts + 31) / 32) * col_groups, nullptr};
-    // persistent: two workgroups per CU, a multiple of the 8 XCDs
-    const uint32_t blocks = std::min<uint32_t>(((uint32_t)ctx->num_cus * 2 + 7) / 8 * 8,
-                                               (P.num_items + 7) / 8 * 8);
+    uint32_t blocks;
+    const DecryptParams P =
+        decrypt_params(d_in, num_samples, num_cts, d_out, (uint32_t)ctx->num_cus, blocks);
     launch_ring<false>(ctx, P, key->bits, blocks);
     IHE_HIP(hipGetLastError());
     return IHE_OK;
@@ -720,6 +786,35 @@ int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const in
   else if (num_samples <= 256) hipLaunchKernelGGL(k_decrypt<4>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);
   else if (num_samples <= 512) hipLaunchKernelGGL(k_decrypt<8>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);
   else hipLaunchKernelGGL(k_decrypt<16>, grid, block, 0, ctx->stream, d_in, num_cts, num_samples, key->bits, d_out);
+  IHE_HIP(hipGetLastError());
+  return IHE_OK;
+}
+
+int ihe_decrypt_noalloc(const int32_t *key, uint32_t num_samples, const int32_t *d_in,
+                        uint32_t num_cts, float *d_out, void *hip_stream) {
+  if (!key) return fail(IHE_ERR_ARG, "null key");
+  if (num_samples == 0 || num_samples > kN) return fail(IHE_ERR_ARG, "NUM_SAMPLES must be in [1, 1024]");
+  if (num_cts > kNoallocMaxCts)
+    return fail(IHE_ERR_ARG, "ihe_decrypt_noalloc takes at most 2^26 ciphertexts per call, got " +
+                                 std::to_string(num_cts));
+  KeyBits bits;
+  std::memset(&bits, 0, sizeof(bits));
+  for (uint32_t i = 0; i < kN; ++i) {
+    if (key[i] != 0 && key[i] != 1) return fail(IHE_ERR_ARG, "TLWE key must be binary");
+    if (key[i]) bits.w[i / 32] |= 1u << (i % 32);
+  }
+  if (num_cts == 0) return IHE_OK;
+  if (!d_in || !d_out) return fail(IHE_ERR_ARG, "null buffer");
+  int dev = 0;
+  IHE_HIP(hipGetDevice(&dev));
+  NoallocDevice D;
+  if (int rc = noalloc_device(dev, D)) return rc;
+  hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+  uint32_t blocks;
+  const DecryptParams P = decrypt_params(d_in, num_samples, num_cts, d_out, D.num_cus, blocks);
+  // the counters start at zero (a fill kernel: no memset node under capture)
+  hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, stream, D.scratch, kRingQueueWords);
+  launch_ring_on<false>(stream, P, bits, blocks, D.scratch);
   IHE_HIP(hipGetLastError());
   return IHE_OK;
 }

@@ -140,6 +140,14 @@ int ihe_ctx_create(int device, ihe_ctx **out) {
     delete c;
     return hip_fail(e, "noise table upload");
   }
+  // the device address of ihe_decrypt_noalloc's work counters, looked up here
+  // so that its calls (also under graph capture) make none
+  if (int rc = noalloc_setup(device)) {
+    (void)hipFree(c->d_noise_table);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return IHE_OK;
 }

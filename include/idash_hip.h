@@ -294,6 +294,21 @@ int ihe_encrypt(ihe_ctx *ctx, const ihe_key *key, const ihe_layout *layout, uint
 int ihe_decrypt(ihe_ctx *ctx, const ihe_key *key, uint32_t num_samples, const int32_t *d_in,
                 uint32_t num_cts, float *d_out);
 
+/* The same decryption as ihe_decrypt, allocating nothing: no context, no
+ * ihe_key, no device or host allocation. It has no counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). key: host
+ * [N] binary key; d_in, d_out as for ihe_decrypt; enqueued on hip_stream
+ * (NULL: the default stream) of the current device. The ring kernel's work
+ * counters (16 words) live in a device array of the library, one per GPU:
+ * calls that overlap on one GPU race on them and must be ordered (one stream,
+ * or events between streams). At most 2^26 ciphertexts per call
+ * (IHE_ERR_ARG beyond). The array's address is looked up when a context is
+ * created on the device (ihe_ctx_create) or at the first call there: make
+ * that first call, or create a context, before capturing a graph. */
+int ihe_decrypt_noalloc(const int32_t *key, uint32_t num_samples, const int32_t *d_in,
+                        uint32_t num_cts, float *d_out, void *hip_stream);
+
 /* Noise table of the encrypt kernel: table[k] = P(|e| <= k) * 2^63 for
  * e = trunc(128 g), g ~ N(0,1) (TFHE gaussian32 at alpha = 2^-25). */
 #define IHE_NOISE_TABLE_LEN 1280u

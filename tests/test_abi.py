@@ -226,3 +226,17 @@ def test_server_checks_request_sizes_before_the_library():
             srv.check_request(*args)
     with pytest.raises(ValueError):
         srv.submit(0, geno, key, 1, out[:-7], num_samples=7)
+
+
+def test_decrypt_noalloc_checks_arguments_before_the_device():
+    """ihe_decrypt_noalloc validates its arguments before any HIP call: these
+    return IHE_ERR_ARG on a machine without a GPU too."""
+    L = _abi.lib()
+    key = np.zeros(1024, np.int32)
+    bad = np.full(1024, 2, np.int32)
+    for args in [(None, 16, 1, 1, 1, None), (key.ctypes.data, 0, 1, 1, 1, None),
+                 (key.ctypes.data, 1025, 1, 1, 1, None),
+                 (key.ctypes.data, 16, 1, (1 << 26) + 1, 1, None),
+                 (bad.ctypes.data, 16, 1, 1, 1, None),
+                 (key.ctypes.data, 16, None, 4, None, None)]:
+        assert L.ihe_decrypt_noalloc(*args) == _abi.IHE_ERR_ARG
