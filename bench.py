@@ -320,6 +320,82 @@ def cli_pipeline(workload, log_fn, workdir=None):
     return {"workload": workload, "figures": figures, "file_bytes": sizes}
 
 
+def _csv_rows(directory: str, suffix: str):
+    """Rows of the rocprofv3 CSV files under `directory` whose names end in `suffix`."""
+    import csv
+    import glob
+    rows = []
+    for path in sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True)):
+        with open(path, newline="") as f:
+            rows += list(csv.DictReader(f))
+    return rows
+
+
+def _counter_per_dispatch(directory: str, counter: str, kernel: str):
+    """{kernel name: [value per dispatch]} of one counter (KiB for FETCH_SIZE and
+    WRITE_SIZE), for the kernels whose name contains `kernel`."""
+    per = {}
+    for r in _csv_rows(directory, "counter_collection.csv"):
+        if r.get("Counter_Name") != counter or kernel not in r.get("Kernel_Name", ""):
+            continue
+        key = (r["Kernel_Name"], r.get("Dispatch_Id", len(per)))
+        per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+    out = {}
+    for (name, _), v in sorted(per.items(), key=lambda kv: int(kv[0][1]) if str(kv[0][1]).isdigit()
+                               else 0):
+        out.setdefault(name, []).append(v)
+    return out
+
+
+def profile_summary(args) -> str:
+    """Text summaries of rocprofv3 runs for profiles/: `trace` (per-kernel call
+    counts and durations from --kernel-trace --stats) or `traffic` (HBM bytes per
+    launch of one kernel from FETCH_SIZE and WRITE_SIZE runs, each scaled by what
+    the counter reads for the 1 GiB device copies of --calibrate-copy)."""
+    lines = [f"# {args.title}", "#", "# Commands (from /tmp with TMPDIR=/tmp; REPO = the repository root):"]
+    lines += [f"#   {c}" for c in args.cmd]
+    lines.append(f"# source_hash of the library sources profiled: {source_hash()}")
+    if args.profile_summary == "trace":
+        rows = _csv_rows(args.trace, "kernel_stats.csv")
+        total = sum(float(r["TotalDurationNs"]) for r in rows) or 1.0
+        lines += ["#", f"{'kernel':<72} {'calls':>6} {'avg_us':>9} {'min_us':>9} {'max_us':>9} {'pct':>6}"]
+        for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+            name = r["Name"].replace("ihe::", "").replace("(anonymous namespace)::", "")
+            lines.append(f"{name[:72]:<72} {int(r['Calls']):>6} {float(r['AverageNs']) / 1e3:>9.1f} "
+                         f"{float(r['MinNs']) / 1e3:>9.1f} {float(r['MaxNs']) / 1e3:>9.1f} "
+                         f"{100 * float(r['TotalDurationNs']) / total:>6.2f}")
+        return "\n".join(lines) + "\n"
+    # traffic
+    copy_r = _counter_per_dispatch(args.copy_fetch, "FETCH_SIZE", "copy")
+    copy_w = _counter_per_dispatch(args.copy_write, "WRITE_SIZE", "copy")
+    (cname_r, cr), = copy_r.items()
+    (cname_w, cw), = copy_w.items()
+    gib_kib = float(1 << 20)                  # one 1 GiB copy reads and writes 2^20 KiB
+    scale_r = round(gib_kib / (sum(cr) / len(cr)), 3)
+    scale_w = round(gib_kib / (sum(cw) / len(cw)), 3)
+    kr = _counter_per_dispatch(args.fetch, "FETCH_SIZE", args.kernel)
+    kw = _counter_per_dispatch(args.write, "WRITE_SIZE", args.kernel)
+    (kname, fr), = kr.items()
+    (_, fw), = kw.items()
+    read_b = sum(fr) / len(fr) * 1024 * scale_r
+    write_b = sum(fw) / len(fw) * 1024 * scale_w
+    alg = int(args.algorithmic_bytes)
+    lines += ["#",
+              f"# calibration: {cname_r.strip()} FETCH_SIZE per dispatch (KiB): "
+              + ", ".join(f"{v:.1f}" for v in cr) + f" -> read scale {scale_r}",
+              f"# calibration: {cname_w.strip()} WRITE_SIZE per dispatch (KiB): "
+              + ", ".join(f"{v:.1f}" for v in cw) + f" -> write scale {scale_w}",
+              f"# kernel: {kname.strip()}",
+              "# FETCH_SIZE per dispatch (KiB): " + ", ".join(f"{v:.2f}" for v in fr),
+              "# WRITE_SIZE per dispatch (KiB): " + ", ".join(f"{v:.2f}" for v in fw),
+              f"# HBM bytes per launch: read {read_b:.0f}, write {write_b:.0f}",
+              f"# algorithmic bytes per launch: {alg}",
+              f"# traffic / algorithmic = {(read_b + write_b) / alg:.4f}",
+              f"source_hash: {source_hash()}",
+              f"traffic_bytes_per_launch: {read_b + write_b:.0f}"]
+    return "\n".join(lines) + "\n"
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -369,7 +445,22 @@ def main(argv=None):
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (a kernel of known "
                          "traffic for scaling HBM counters) and exit")
+    ap.add_argument("--profile-summary", choices=["trace", "traffic"],
+                    help="only print a profiles/ summary of rocprofv3 CSV output (no GPU use)")
+    ap.add_argument("--title", default="rocprofv3 summary")
+    ap.add_argument("--cmd", action="append", default=[], help="command that made the data")
+    ap.add_argument("--trace", help="directory of a --kernel-trace --stats run")
+    ap.add_argument("--fetch", help="directory of the FETCH_SIZE run")
+    ap.add_argument("--write", help="directory of the WRITE_SIZE run")
+    ap.add_argument("--copy-fetch", help="directory of the FETCH_SIZE run of --calibrate-copy")
+    ap.add_argument("--copy-write", help="directory of the WRITE_SIZE run of --calibrate-copy")
+    ap.add_argument("--kernel", default="k_cloud", help="kernel name part to sum counters of")
+    ap.add_argument("--algorithmic-bytes", type=int, default=0)
     args = ap.parse_args(argv)
+
+    if args.profile_summary:
+        sys.stdout.write(profile_summary(args))
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the cloud step "
