@@ -355,6 +355,21 @@ def profile_summary(args) -> str:
     lines = [f"# {args.title}", "#", "# Commands (from /tmp with TMPDIR=/tmp; REPO = the repository root):"]
     lines += [f"#   {c}" for c in args.cmd]
     lines.append(f"# source_hash of the library sources profiled: {source_hash()}")
+    if args.profile_summary == "counters":
+        # every counter of the listed runs, averaged over the kernel's dispatches
+        vals = {}
+        for d in args.dirs:
+            for r in _csv_rows(d, "counter_collection.csv"):
+                if args.kernel not in r.get("Kernel_Name", ""):
+                    continue
+                key = (r["Kernel_Name"], r["Counter_Name"])
+                vals.setdefault(key, {}).setdefault(r.get("Dispatch_Id"), 0.0)
+                vals[key][r.get("Dispatch_Id")] += float(r["Counter_Value"])
+        lines += ["#", f"{'kernel':<56} {'counter':<30} {'dispatches':>10} {'per dispatch':>16}"]
+        for (k, c), per in sorted(vals.items()):
+            name = k.replace("ihe::", "").replace("(anonymous namespace)::", "")
+            lines.append(f"{name[:56]:<56} {c:<30} {len(per):>10} {sum(per.values()) / len(per):>16.1f}")
+        return "\n".join(lines) + "\n"
     if args.profile_summary == "trace":
         rows = _csv_rows(args.trace, "kernel_stats.csv")
         total = sum(float(r["TotalDurationNs"]) for r in rows) or 1.0
@@ -445,11 +460,12 @@ def main(argv=None):
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (a kernel of known "
                          "traffic for scaling HBM counters) and exit")
-    ap.add_argument("--profile-summary", choices=["trace", "traffic"],
+    ap.add_argument("--profile-summary", choices=["trace", "traffic", "counters"],
                     help="only print a profiles/ summary of rocprofv3 CSV output (no GPU use)")
     ap.add_argument("--title", default="rocprofv3 summary")
     ap.add_argument("--cmd", action="append", default=[], help="command that made the data")
     ap.add_argument("--trace", help="directory of a --kernel-trace --stats run")
+    ap.add_argument("--dirs", nargs="*", default=[], help="directories of --pmc runs (counters)")
     ap.add_argument("--fetch", help="directory of the FETCH_SIZE run")
     ap.add_argument("--write", help="directory of the WRITE_SIZE run")
     ap.add_argument("--copy-fetch", help="directory of the FETCH_SIZE run of --calibrate-copy")

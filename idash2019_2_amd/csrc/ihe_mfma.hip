@@ -165,10 +165,25 @@ __device__ __forceinline__ void mfma_kblock(const int4v &a0, const int4v &a1, co
   c.s24 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pl[2], c.s24, 0, 0, 0);
 }
 
+// (a << 8) + b, kept as one v_lshl_add_u32: the empty asm makes the result
+// opaque, so the recombination below stays a chain of three per word (the
+// compiler otherwise reassociates it into separate shifts of each accumulator
+// and an add3, four per word).
+__device__ __forceinline__ uint32_t shl8_add(uint32_t a, uint32_t b) {
+  uint32_t d = (a << 8) + b;
+  asm("" : "+v"(d));
+  return d;
+}
+
 // Recombines, applies the b-polynomial bias (samples < NS) and hiding (words
 // >= RS), and stores 32 rows x 32 words: row m of lanes 0..31 (and m + 4 of
 // lanes 32..63) for value v, m = 8*(v/4) + v%4. Rows at or past nrows store
-// out of range. Always 16 stores.
+// out of range. Always 16 stores. n0 (the column tile's first word) is
+// wave-uniform, so every case below is a scalar branch: whole column tiles
+// inside NS take plain adds, and full tiles store without selects, when SPLIT
+// (the per-wave kernels; the 128-row kernel keeps the selects, a single store
+// sequence and its row base as computed: its register budget needs them).
+template <bool SPLIT>
 __device__ __forceinline__ void finish_rows(const Acc &c, const MfmaEntry &E,
                                             __amdgpu_buffer_rsrc_t out_r, const uint32_t *bias,
                                             uint32_t n0, uint32_t r, uint32_t h, uint32_t row0,
@@ -177,31 +192,51 @@ __device__ __forceinline__ void finish_rows(const Acc &c, const MfmaEntry &E,
   uint32_t o[16];
 #pragma unroll
   for (int v = 0; v < 16; ++v)  // s0 + 2^8 s8 + 2^16 s16 + 2^24 s24, Horner form
-    o[v] = ((((((uint32_t)c.s24[v] << 8) + (uint32_t)c.s16[v]) << 8) + (uint32_t)c.s8[v]) << 8) +
-           (uint32_t)c.s0[v];
-  // uniform per column tile except where NS or RS falls inside it
-  const bool b_poly = n0 >= kN;
-  if (b_poly && n0 - kN < E.ns) {
-    const bool lane_bias = word - kN < E.ns;
+    o[v] = shl8_add(shl8_add(shl8_add((uint32_t)c.s24[v], (uint32_t)c.s16[v]), (uint32_t)c.s8[v]),
+                    (uint32_t)c.s0[v]);
+  if (n0 >= kN && n0 - kN < E.ns) {
+    if (SPLIT && n0 + 31 - kN < E.ns) {  // every word of the tile takes the bias
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const u32x4v bv = *(const u32x4v *)(bias + row0 + 8 * g + 4 * h);
+      for (int g = 0; g < 4; ++g) {
+        const u32x4v bv = *(const u32x4v *)(bias + row0 + 8 * g + 4 * h);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) o[4 * g + q] += lane_bias ? bv[q] : 0u;
+        for (int q = 0; q < 4; ++q) o[4 * g + q] += bv[q];
+      }
+    } else {
+      const bool lane_bias = word - kN < E.ns;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const u32x4v bv = *(const u32x4v *)(bias + row0 + 8 * g + 4 * h);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o[4 * g + q] += lane_bias ? bv[q] : 0u;
+      }
     }
   }
-  if (b_poly && n0 + 31 - kN >= E.rs) {
+  if (n0 >= kN && n0 + 31 - kN >= E.rs) {
     const bool keep = word - kN < E.rs;
 #pragma unroll
     for (int v = 0; v < 16; ++v) o[v] = keep ? o[v] : 0u;
   }
   const uint32_t voff = word * 4u + 4 * h * kCtWords * 4u;
+  // the row block's byte offset, kept scalar: each store's offset is then this
+  // SGPR plus a constant
+  uint32_t sbase = rbase + row0 * kCtWords * 4u;
+  if constexpr (SPLIT) asm volatile("" : "+s"(sbase));
+  if (SPLIT && full) {
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const uint32_t m = row0 + 8 * (v / 4) + (v % 4);
-    __builtin_amdgcn_raw_buffer_store_b32((int)o[v], out_r,
-                                          full || m + 4 * h < nrows ? voff : kDropOffset,
-                                          (int)(rbase + m * kCtWords * 4u), kStorePolicy);
+    for (int v = 0; v < 16; ++v) {
+      const uint32_t m = 8 * (v / 4) + (v % 4);
+      __builtin_amdgcn_raw_buffer_store_b32((int)o[v], out_r, voff,
+                                            (int)(sbase + m * kCtWords * 4u), kStorePolicy);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const uint32_t m = 8 * (v / 4) + (v % 4);
+      __builtin_amdgcn_raw_buffer_store_b32((int)o[v], out_r,
+                                            full || row0 + m + 4 * h < nrows ? voff : kDropOffset,
+                                            (int)(sbase + m * kCtWords * 4u), kStorePolicy);
+    }
   }
 }
 
@@ -223,7 +258,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfma1(MfmaArgs A) {
   uint32_t tile;
   const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
   const uint32_t part = blk % wg_per_tile;
-  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  // the wave index is uniform: column-tile positions and their branches stay scalar
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
   const uint32_t h = lane / 32, r = lane % 32;
   const MTileDev T = tile_of(E, tile);
   const __amdgpu_buffer_rsrc_t in_r =
@@ -263,7 +299,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfma1(MfmaArgs A) {
     for (int rb = 0; rb < RB; ++rb) {
       Acc c = {};
       mfma_kblock(a[rb][0], a[rb][1], pl, c);
-      finish_rows(c, E, out_r, s_bias, col(nt) * 32, r, h, 32 * rb, T.nrows, full, rbase);
+      finish_rows<true>(c, E, out_r, s_bias, col(nt) * 32, r, h, 32 * rb, T.nrows, full, rbase);
     }
   };
 
@@ -297,7 +333,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
   uint32_t tile;
   const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
   const uint32_t part = blk % wg_per_tile;
-  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  // the wave index is uniform: column-tile positions and their branches stay scalar
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
   const uint32_t h = lane / 32, r = lane % 32;
   const MTileDev T = tile_of(E, tile);
   const __amdgpu_buffer_rsrc_t in_r =
@@ -360,7 +397,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
     }
     uint32_t rbase = row_base;
     asm volatile("" : "+s"(rbase));
-    finish_rows(c, E, out_r, s_bias, col(nt) * 32, r, h, 0, T.nrows, full, rbase);
+    finish_rows<true>(c, E, out_r, s_bias, col(nt) * 32, r, h, 0, T.nrows, full, rbase);
 #pragma unroll
     for (int j = 0; j < 16; ++j) xa[j] = xnext[j];
   };
@@ -515,7 +552,7 @@ void k_cloud_mfmaw(MfmaArgs A) {
     }
     uint32_t rbase = row_base;
     asm volatile("" : "+s"(rbase));
-    finish_rows(c, E, out_r, s_bias, col(nt) * 32, r, h, 32 * wave, T.nrows, full, rbase);
+    finish_rows<false>(c, E, out_r, s_bias, col(nt) * 32, r, h, 32 * wave, T.nrows, full, rbase);
     __syncthreads();
   };
   // the first column tile is peeled so that the loop is entered, like it is
