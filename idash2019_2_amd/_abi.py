@@ -48,7 +48,7 @@ class Cohort(ctypes.Structure):
 
 class BatchInfo(ctypes.Structure):
     _fields_ = [("num_cohorts", _u32), ("num_launches", _u32), ("num_tiles", _u32),
-                ("captured", _u32)]
+                ("captured", _u32), ("persistent", _u32)]
 
 
 class TuneEntry(ctypes.Structure):

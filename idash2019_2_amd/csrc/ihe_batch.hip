@@ -1,14 +1,22 @@
 // Batched cloud step: many cohorts (each with its own sample count, input
-// ciphertexts and resident model band) in one launch per MFMA kernel family
-// plus one variance launch, optionally captured into a hipGraph so that a
-// step is a single graph launch.
+// ciphertexts and resident model band) in one persistent launch, optionally
+// captured into a hipGraph so that a step is a single graph launch.
 //
 // The reference runs one cloud_compute_score per cohort (eval/cloud.cpp:17);
-// here the per-cohort work lists are concatenated. A launch covers every
-// cohort whose band uses the same kernel family; its workgroups find their
-// (cohort segment, tile) by a binary search over the family's entry table,
-// and consecutive workgroups of an XCD walk consecutive tiles of one cohort.
+// here the per-cohort work lists are concatenated. Cohorts whose bands share
+// a kernel family (tile shape and rotation) form one family: its work is a
+// list of workgroup blocks, each finding its (cohort segment, tile) by a
+// lookup in a tile -> entry table. The one kernel (k_cloud_batch) runs the
+// one-K-block families' blocks first, one per workgroup as the dispatcher
+// hands them out, and then its resident workgroups take the other families'
+// blocks and the variance blocks from per-XCD counters; XCD x starts on its
+// own contiguous share of that list (shares cut by estimated time), so
+// consecutive blocks of one cohort meet in one L2. A batch with a family the
+// kernel has no body for (or with IHE_BATCH_PERSISTENT=0) runs one launch
+// per family instead, the variance blocks joining the last.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -23,14 +31,19 @@ struct ihe_batch {
     MfmaShape shape;
     bool rot;
     uint32_t entry0, num_entries, first0, num_items;
+    uint32_t item0;  // first tile in the tile -> entry table
   };
   std::vector<Launch> launches;
+  bool persistent = false;
+  PersistArgs P{};
+  uint32_t pgrid = 0;
   std::vector<const ihe_band *> bands;  // each distinct band once; its use count is held
   uint32_t num_cohorts = 0, num_items = 0;
   uint32_t num_ventries = 0, num_vblocks = 0;
   void *dmem = nullptr;
   MfmaEntry *d_entries = nullptr;
   uint32_t *d_first = nullptr;
+  uint32_t *d_item_entry = nullptr;  // per family launch: entry of each tile
   VarEntry *d_ventries = nullptr;
   uint32_t *d_vfirst = nullptr;
   hipGraph_t graph = nullptr;
@@ -52,18 +65,37 @@ bool same_family(const Keyed &a, const Keyed &b) {
          a.rot == b.rot;
 }
 
-void enqueue(const ihe_batch *b, hipStream_t s) {
-  MfmaEntry none = {};
+VarArgs var_args(const ihe_batch *b) {
   VarArgs V{};
   V.entries = b->d_ventries;
   V.first = b->d_vfirst;
   V.num_entries = b->num_ventries;
   V.num_blocks = b->num_ventries ? b->num_vblocks : 0;
+  return V;
+}
+
+// Relative time per output word of a persistent body (one-K-block tiles run
+// the full 16,184 / 80,882 layout in about 0.47 ms, 128-row tiles of 2, 4 and
+// 6 K-blocks in about 0.55, 0.68 and 0.88 ms), and the words of its blocks.
+double body_cost(uint32_t body) {
+  if (body < 4) return 0.47;
+  const uint32_t k = (body - 4) % 3;
+  return k == 0 ? 0.55 : k == 1 ? 0.68 : 0.88;
+}
+double block_words(const MfmaShape &shape) {
+  return (double)kCtWords * kMfmaRows * shape.rb / persist_wg_per_tile(shape);
+}
+
+void enqueue(const ihe_batch *b, hipStream_t s) {
+  if (b->persistent) return launch_persistent(s, b->P, b->pgrid);
+  MfmaEntry none = {};
+  const VarArgs V = var_args(b);
   // the variance blocks join the last family's launch
   for (size_t i = 0; i < b->launches.size(); ++i) {
     const ihe_batch::Launch &L = b->launches[i];
     launch_mfma(s, L.shape, L.rot, none, b->d_entries + L.entry0, b->d_first + L.first0,
-                L.num_entries, L.num_items, i + 1 == b->launches.size() ? V : VarArgs{});
+                L.num_entries, L.num_items, i + 1 == b->launches.size() ? V : VarArgs{},
+                b->d_item_entry + L.item0);
   }
   if (b->launches.empty()) launch_var(s, V);
 }
@@ -142,16 +174,17 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
   b->ctx = ctx;
   b->num_cohorts = num_cohorts;
   std::vector<MfmaEntry> entries;
-  std::vector<uint32_t> first;
+  std::vector<uint32_t> first, item_entry;
   for (size_t i = 0; i < keyed.size();) {
     size_t j = i;
     ihe_batch::Launch L{keyed[i].shape, keyed[i].rot, (uint32_t)entries.size(), 0,
-                        (uint32_t)first.size(), 0};
+                        (uint32_t)first.size(), 0, (uint32_t)item_entry.size()};
     uint32_t items = 0;
     for (; j < keyed.size() && same_family(keyed[i], keyed[j]); ++j) {
       entries.push_back(keyed[j].e);
       first.push_back(items);
       items += keyed[j].tiles;
+      item_entry.insert(item_entry.end(), keyed[j].tiles, (uint32_t)(j - i));
     }
     first.push_back(items);
     L.num_entries = (uint32_t)(j - i);
@@ -162,11 +195,27 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
   }
   b->num_ventries = (uint32_t)vent.size();
   b->num_vblocks = vfirst.back();
+  // one persistent launch when every family has a body there
+  const char *pe = std::getenv("IHE_BATCH_PERSISTENT");
+  b->persistent = !b->launches.empty() && !(pe && std::strcmp(pe, "0") == 0);
+  for (const ihe_batch::Launch &L : b->launches)
+    b->persistent = b->persistent && persist_body(L.shape, L.rot) >= 0;
+  if (b->persistent) {
+    if (int rc = persist_grid(ctx->device, b->pgrid)) {
+      delete b;
+      return rc;
+    }
+  }
+  const size_t num_fams = b->persistent ? b->launches.size() : 0;  // one-shot and resident
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_first = align(entries.size() * sizeof(MfmaEntry));
   const size_t o_vent = o_first + align(first.size() * 4);
   const size_t o_vfirst = o_vent + align(vent.size() * sizeof(VarEntry));
-  const size_t bytes = o_vfirst + align(vfirst.size() * 4);
+  const size_t o_items = o_vfirst + align(vfirst.size() * 4);
+  const size_t o_fams = o_items + align(item_entry.size() * 4);
+  const size_t o_ffirst = o_fams + align(num_fams * sizeof(PersistFamily));
+  const size_t o_queue = o_ffirst + align((num_fams + 2) * 4);
+  const size_t bytes = o_queue + (b->persistent ? align(kPersistQueueWords * 4) : 0);
   if (int rc = ihe_malloc(ctx, bytes, &b->dmem)) {
     delete b;
     return rc;
@@ -176,11 +225,80 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
   if (!first.empty()) std::memcpy(h.data() + o_first, first.data(), first.size() * 4);
   if (!vent.empty()) std::memcpy(h.data() + o_vent, vent.data(), vent.size() * sizeof(VarEntry));
   std::memcpy(h.data() + o_vfirst, vfirst.data(), vfirst.size() * 4);
+  if (!item_entry.empty())
+    std::memcpy(h.data() + o_items, item_entry.data(), item_entry.size() * 4);
   char *d = static_cast<char *>(b->dmem);
   b->d_entries = reinterpret_cast<MfmaEntry *>(d);
   b->d_first = reinterpret_cast<uint32_t *>(d + o_first);
   b->d_ventries = reinterpret_cast<VarEntry *>(d + o_vent);
   b->d_vfirst = reinterpret_cast<uint32_t *>(d + o_vfirst);
+  b->d_item_entry = reinterpret_cast<uint32_t *>(d + o_items);
+  if (b->persistent) {
+    // the one-K-block families first (their blocks lead the grid), then the
+    // others in launch order (kPersistBlocks workgroup blocks per item) and
+    // the variance blocks, cut into XCD shares by estimated time (a variance
+    // block taken as 1/16 of a 64-row block)
+    std::vector<PersistFamily> fams;
+    std::vector<const ihe_batch::Launch *> order;
+    for (int pass = 0; pass < 2; ++pass)
+      for (const ihe_batch::Launch &L : b->launches)
+        if ((persist_body(L.shape, L.rot) < 4) == (pass == 0)) order.push_back(&L);
+    std::vector<uint32_t> once_first(1, 0), ffirst(1, 0);
+    std::vector<double> cost;
+    double total_cost = 0;
+    for (const ihe_batch::Launch *L : order) {
+      PersistFamily F{};
+      F.args.entries = b->d_entries + L->entry0;
+      F.args.first = b->d_first + L->first0;
+      F.args.item_entry = b->d_item_entry + L->item0;
+      F.args.num_entries = L->num_entries;
+      F.args.cloud_blocks = L->num_items * persist_wg_per_tile(L->shape);
+      F.body = (uint32_t)persist_body(L->shape, L->rot);
+      if (F.body < 4) {
+        once_first.push_back(once_first.back() + F.args.cloud_blocks);
+      } else {
+        const uint32_t items = (F.args.cloud_blocks + kPersistBlocks - 1) / kPersistBlocks;
+        ffirst.push_back(ffirst.back() + items);
+        cost.push_back(body_cost(F.body) * block_words(L->shape) * kPersistBlocks);
+        total_cost += cost.back() * items;
+      }
+      fams.push_back(F);
+    }
+    const size_t num_res = ffirst.size() - 1;
+    const double var_cost = 0.47 * kCtWords * kMfmaRows * 2 / 4 / 16;
+    total_cost += var_cost * b->num_vblocks;
+    PersistArgs &P = b->P;
+    P.fams = reinterpret_cast<const PersistFamily *>(d + o_fams);
+    P.once_first = reinterpret_cast<const uint32_t *>(d + o_ffirst);
+    P.fam_first = P.once_first + once_first.size();
+    P.num_once = (uint32_t)(once_first.size() - 1);
+    P.once_blocks = once_first.back();
+    P.num_fams = (uint32_t)num_res;
+    P.total = ffirst.back() + b->num_vblocks;
+    P.queue = reinterpret_cast<uint32_t *>(d + o_queue);
+    P.var = var_args(b);
+    // bounds[x] = first item whose running cost reaches x / 8 of the total
+    P.bounds[0] = 0;
+    P.bounds[8] = P.total;
+    double run = 0;
+    uint32_t x = 1;
+    for (size_t f = 0; f <= num_res && x < 8; ++f) {
+      const uint32_t n = f < num_res ? ffirst[f + 1] - ffirst[f] : b->num_vblocks;
+      const double c = f < num_res ? cost[f] : var_cost;
+      const uint32_t base = ffirst[f];
+      while (x < 8 && run + c * n >= total_cost * x / 8) {
+        const double need = std::max(0.0, total_cost * x / 8 - run);
+        P.bounds[x] = base + std::min<uint32_t>(n, (uint32_t)std::ceil(need / c));
+        ++x;
+      }
+      run += c * n;
+    }
+    for (; x < 8; ++x) P.bounds[x] = P.total;
+    for (x = 1; x <= 8; ++x) P.bounds[x] = std::max(P.bounds[x], P.bounds[x - 1]);
+    std::memcpy(h.data() + o_fams, fams.data(), fams.size() * sizeof(PersistFamily));
+    std::memcpy(h.data() + o_ffirst, once_first.data(), once_first.size() * 4);
+    std::memcpy(h.data() + o_ffirst + once_first.size() * 4, ffirst.data(), ffirst.size() * 4);
+  }
   if (int rc = ihe_memcpy_h2d(ctx, b->dmem, h.data(), bytes)) {
     (void)ihe_free(ctx, b->dmem);
     delete b;
@@ -224,7 +342,10 @@ int ihe_batch_capture(ihe_batch *b) {
 int ihe_batch_get_info(const ihe_batch *b, ihe_batch_info *info) {
   if (!b || !info) return fail(IHE_ERR_ARG, "null argument");
   info->num_cohorts = b->num_cohorts;
-  info->num_launches = (uint32_t)b->launches.size() + (b->launches.empty() && b->num_ventries ? 1u : 0u);
+  info->num_launches = b->persistent ? 1u
+                                      : (uint32_t)b->launches.size() +
+                                            (b->launches.empty() && b->num_ventries ? 1u : 0u);
+  info->persistent = b->persistent ? 1u : 0u;
   info->num_tiles = b->num_items;
   info->captured = b->exec ? 1u : 0u;
   return IHE_OK;

@@ -106,14 +106,66 @@ struct VarArgs {
 
 void launch_var(hipStream_t stream, const VarArgs &var);
 
+// Arguments of one family launch: a single cohort (`entries` null: `single`)
+// or a table of cohort segments (first[e] = entry e's first tile, and
+// optionally item_entry[t] = the entry of tile t); blocks at or past
+// cloud_blocks run the variance blocks of `var`.
+struct MfmaArgs {
+  MfmaEntry single;
+  const MfmaEntry *entries;
+  const uint32_t *first;
+  const uint32_t *item_entry;
+  uint32_t num_entries;
+  uint32_t cloud_blocks;
+  VarArgs var;
+};
+
+// Persistent batch (k_cloud_batch): the families whose bodies it runs, with
+// its own column-tile counts (NTW does not change a band's tiling).
+constexpr int kPersistNtw1 = 4;          // per wave, one-K-block tiles of 32 or 64 rows
+constexpr int kPersistNtwW = 16;         // per workgroup, 128-row tiles of up to 4 K-blocks
+constexpr int kPersistNtwW6 = 32;        // per workgroup, 128-row tiles of 5 or 6 K-blocks
+constexpr uint32_t kPersistMaxNkb = 6;   // 128-row tiles of up to 6 K-blocks
+constexpr uint32_t kPersistQueueWords = 16;  // 8 XCD counters, finished workgroups
+constexpr uint32_t kPersistBlocks = 2;       // workgroup blocks per item
+
+struct alignas(16) PersistFamily {
+  MfmaArgs args;  // no variance work (var.num_blocks = 0)
+  uint32_t body;  // persist_body of the family
+};
+
+struct PersistArgs {
+  const PersistFamily *fams;  // the num_once one-K-block families, then the num_fams others
+  const uint32_t *once_first; // [num_once + 1]: first workgroup block of each one-K-block family
+  const uint32_t *fam_first;  // [num_fams + 1]: first item of each other family (kPersistBlocks
+                              // blocks per item); then the variance items (one block each)
+  uint32_t num_once, once_blocks;  // once_blocks = once_first[num_once]: the grid's first blocks
+  uint32_t num_fams, total;   // total = items of the resident workgroups
+  uint32_t *queue;            // kPersistQueueWords words, zero before the first launch
+  uint32_t bounds[9];         // XCD x's own items: [bounds[x], bounds[x + 1])
+  VarArgs var;
+};
+
+// The body of a family in k_cloud_batch, or -1 when it has none (per-wave
+// tiles over several K-blocks, 128-row tiles over 7 or 8).
+int persist_body(const MfmaShape &shape, bool rot);
+// Workgroup blocks per tile of a family in k_cloud_batch.
+uint32_t persist_wg_per_tile(const MfmaShape &shape);
+// Resident workgroups of a k_cloud_batch launch on `device` (all resident at
+// once); the grid is once_blocks + this.
+int persist_grid(int device, uint32_t &grid);
+void launch_persistent(hipStream_t stream, const PersistArgs &P, uint32_t grid);
+
 // Enqueues the MFMA cloud kernel. Single cohort: entries == nullptr and
 // `single` describes it, with num_items = its tiles. Batch: entries[e] for
 // e < num_entries, first[e] = index of entry e's first tile in the launch
 // (first[num_entries] = num_items). `var` (num_blocks > 0) appends its
 // variance blocks to the same grid: the step is one launch, and the variance
 // blocks fill the cloud blocks' drain tail.
+// item_entry (batches, optional): the entry of each of the num_items tiles,
+// in place of a search over `first`.
 void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const MfmaEntry &single,
                  const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
-                 uint32_t num_items, const VarArgs &var);
+                 uint32_t num_items, const VarArgs &var, const uint32_t *item_entry = nullptr);
 
 }  // namespace ihe

@@ -41,16 +41,8 @@ __device__ __forceinline__ const IHE_CONSTANT T *cptr(const T *p) {
   return (const IHE_CONSTANT T *)p;
 }
 
-struct MfmaArgs {
-  MfmaEntry single;
-  const MfmaEntry *entries;
-  const uint32_t *first;
-  uint32_t num_entries;
-  uint32_t cloud_blocks;  // blocks at or past this index run variance blocks
-  VarArgs var;
-};
-
-__device__ __forceinline__ void var_block(const VarArgs &A, uint32_t chunk);
+__device__ __forceinline__ void var_block(const VarArgs &A, uint32_t chunk,
+                                          uint32_t t = threadIdx.x);
 
 struct EntryWords {
   u32x16v lo, hi;
@@ -64,6 +56,16 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
   return b < 8 * per ? (b % 8) * per + b / 8 : b;
 }
 
+// The thread's index in its workgroup, opaque to the optimizer. The persistent
+// kernel hands it to the bodies per item, so that what they derive from it is
+// computed where it is used, never hoisted out of the item loop into
+// registers that would live (and be spilled) across every body.
+__device__ __forceinline__ uint32_t tid() {
+  uint32_t t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 __device__ __forceinline__ MfmaEntry entry_of(const MfmaArgs &A, uint32_t item, uint32_t &tile) {
   if (!A.entries) {
     tile = A.single.tile_base + item;
@@ -71,10 +73,14 @@ __device__ __forceinline__ MfmaEntry entry_of(const MfmaArgs &A, uint32_t item, 
   }
   const IHE_CONSTANT uint32_t *first = cptr(A.first);
   uint32_t lo = 0, hi = A.num_entries;  // first[lo] <= item < first[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (first[mid] <= item) lo = mid;
-    else hi = mid;
+  if (A.item_entry) {
+    lo = cptr(A.item_entry)[item];
+  } else {
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (first[mid] <= item) lo = mid;
+      else hi = mid;
+    }
   }
   const IHE_CONSTANT u32x16v *e = (const IHE_CONSTANT u32x16v *)(A.entries + lo);
   const EntryWords w = {e[0], e[1]};
@@ -248,25 +254,26 @@ __device__ __forceinline__ void finish_rows(const Acc &c, const MfmaEntry &E,
 // columns; the last step reloads its own) followed by 16*RB stores, so the
 // loads are waited with the stores still in flight; the first step is peeled
 // so the loop is entered with the same pending counts as it is re-entered.
+//
+// Workgroup block `blk` of the launch (its tile and column part), for thread t;
+// s_bias holds 32 RB words of LDS.
 template <bool ROT, int NTW, int RB>
-__global__ __launch_bounds__(64 * kWaves) void k_cloud_mfma1(MfmaArgs A) {
+__device__ __forceinline__ void mfma1_body(const MfmaArgs &A, uint32_t blk, uint32_t *s_bias,
+                                           uint32_t t) {
   constexpr uint32_t wg_per_tile = kCtWords / (kWaves * NTW * 32);
   constexpr uint32_t rows = kMfmaRows * RB;
-  __shared__ uint32_t s_bias[rows];
-  if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
-  const uint32_t blk = xcd_block(blockIdx.x, A.cloud_blocks);
   uint32_t tile;
   const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
   const uint32_t part = blk % wg_per_tile;
   // the wave index is uniform: column-tile positions and their branches stay scalar
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(t / 64), lane = t % 64;
   const uint32_t h = lane / 32, r = lane % 32;
   const MTileDev T = tile_of(E, tile);
   const __amdgpu_buffer_rsrc_t in_r =
       __builtin_amdgcn_make_buffer_rsrc((void *)E.in, (short)0, (int)E.in_bytes, kRsrcWord3);
   const __amdgpu_buffer_rsrc_t out_r =
       __builtin_amdgcn_make_buffer_rsrc((void *)E.out, (short)0, (int)E.out_bytes, kRsrcWord3);
-  if (threadIdx.x < rows) s_bias[threadIdx.x] = E.rowc[T.row_off + threadIdx.x];
+  if (t < rows) s_bias[t] = E.rowc[T.row_off + t];
   uint32_t cw[16];
   {
     const IHE_CONSTANT uint32_t *kc = cptr(E.kcols) + T.kcol_off;
@@ -314,6 +321,13 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfma1(MfmaArgs A) {
     load_cols<ROT>(in_r, cw, col(ntn) * 32 + r, xn);
     step(x, nt);
   }
+}
+
+template <bool ROT, int NTW, int RB>
+__global__ __launch_bounds__(64 * kWaves) void k_cloud_mfma1(MfmaArgs A) {
+  __shared__ uint32_t s_bias[kMfmaRows * RB];
+  if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
+  mfma1_body<ROT, NTW, RB>(A, xcd_block(blockIdx.x, A.cloud_blocks), s_bias, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -434,31 +448,31 @@ __global__ __launch_bounds__(64 * kWaves) void k_cloud_mfmak(MfmaArgs A) {
 template <bool ROT, int NKB>
 constexpr int mfmaw_waves() { return ROT ? (NKB == 6 ? 2 : 1) : (NKB == 4 ? 3 : 1); }
 
+//
+// Workgroup block `blk` of the launch, for thread t; s_b is [buffer][(kb * 4 +
+// plane) * 64 + B lane] and s_bias 128 words of LDS.
 template <bool ROT, int NTW, int NKB>
-__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(mfmaw_waves<ROT, NKB>())))
-void k_cloud_mfmaw(MfmaArgs A) {
+__device__ __forceinline__ void mfmaw_body(const MfmaArgs &A, uint32_t blk,
+                                           int4v (*s_b)[NKB * 4 * 64], uint32_t *s_bias,
+                                           uint32_t t) {
   static_assert(NKB % 2 == 0, "the four waves split NKB / 2 K-block pairs");
   constexpr uint32_t wg_per_tile = kCtWords / (NTW * 32);
   constexpr uint32_t rows = kMfmaRows * kWaves;
   constexpr int kFull = NKB / 4;                    // rounds of two half K-blocks per wave
   constexpr int kRounds = kFull + (NKB % 4 ? 1 : 0);
-  __shared__ int4v s_b[2][NKB * 4 * 64];  // [buffer][(kb * 4 + plane) * 64 + B lane]
-  __shared__ uint32_t s_bias[rows];
-  if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
-  const uint32_t blk = xcd_block(blockIdx.x, A.cloud_blocks);
   uint32_t tile;
   const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
   const uint32_t nt0 = (blk % wg_per_tile) * NTW;
   auto col = [&](uint32_t nt) { return nt0 + nt; };
   // the wave index is uniform: row offsets stay scalar
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(t / 64), lane = t % 64;
   const uint32_t h = lane / 32, r = lane % 32;
   const MTileDev T = tile_of(E, tile);
   const __amdgpu_buffer_rsrc_t in_r =
       __builtin_amdgcn_make_buffer_rsrc((void *)E.in, (short)0, (int)E.in_bytes, kRsrcWord3);
   const __amdgpu_buffer_rsrc_t out_r =
       __builtin_amdgcn_make_buffer_rsrc((void *)E.out, (short)0, (int)E.out_bytes, kRsrcWord3);
-  if (threadIdx.x < rows) s_bias[threadIdx.x] = E.rowc[T.row_off + threadIdx.x];
+  if (t < rows) s_bias[t] = E.rowc[T.row_off + t];
   // this wave's A operands; K-blocks past the tile's own are zero
   int4v a[NKB][2];
 #pragma unroll
@@ -561,6 +575,152 @@ void k_cloud_mfmaw(MfmaArgs A) {
   for (uint32_t nt = 1; nt < (uint32_t)NTW; ++nt) column_tile(nt);
 }
 
+template <bool ROT, int NTW, int NKB>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(mfmaw_waves<ROT, NKB>())))
+void k_cloud_mfmaw(MfmaArgs A) {
+  __shared__ int4v s_b[2][NKB * 4 * 64];
+  __shared__ uint32_t s_bias[kMfmaRows * kWaves];
+  if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
+  mfmaw_body<ROT, NTW, NKB>(A, xcd_block(blockIdx.x, A.cloud_blocks), s_b, s_bias, threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// A whole batch in one launch. The grid's first workgroups each run one block
+// of the one-K-block families (32- or 64-row tiles): those blocks are short
+// and store-bound, and run as in their own kernel when the dispatcher hands
+// them out (taken from a work counter instead, they measured a third slower).
+// The workgroups after them stay resident and take items (kPersistBlocks
+// consecutive workgroup blocks of one 128-row family, or a block of 256
+// output variances) until none are left: XCD x first takes the items of its
+// own contiguous range, from its own counter, so neighbouring tiles
+// (overlapping column windows) meet in that XCD's L2; then it takes from the
+// other XCDs' ranges. They are dispatched as the first ones drain, so the two
+// kinds of work overlap at the switch. Every body runs as in its own kernel;
+// the register and LDS budgets are the largest body's (two workgroups per CU).
+template <int NKB>
+__device__ __forceinline__ int4v (*as_planes(int4v *p))[NKB * 4 * 64] {
+  return reinterpret_cast<int4v (*)[NKB * 4 * 64]>(p);
+}
+
+// The next item from the counter value `got` (lane 0's) of range `victim`:
+// that range's item if it has one left, else the first left in the ranges
+// after it (more increments, by lane 0), else P.total. Run by a whole wave
+// with wave-uniform values, so the bounds are scalar loads: vector loads
+// here, issued behind the item's stores, would wait for all of them.
+__device__ __forceinline__ uint32_t resolve_item(const PersistArgs &P, uint32_t xcd,
+                                                 uint32_t &victim, uint32_t got) {
+  uint32_t g = __builtin_amdgcn_readfirstlane(got);
+  for (;;) {
+    const uint32_t v = P.bounds[victim] + g;
+    if (v < P.bounds[victim + 1]) return v;
+    victim = (victim + 1) % 8;
+    if (victim == xcd) return P.total;
+    uint32_t a = 0;
+    if (tid() % 64 == 0) a = atomicAdd(&P.queue[victim], 1u);
+    g = __builtin_amdgcn_readfirstlane(a);
+  }
+}
+
+// A family's arguments read through the constant address space: scalar
+// loads, so the buffer descriptors the bodies build from them stay scalar
+// (read as generic memory, every buffer access of a body took a loop over the
+// lanes' descriptors).
+__device__ __forceinline__ MfmaArgs family_args(const IHE_CONSTANT PersistFamily *F) {
+  MfmaArgs a{};
+  a.entries = F->args.entries;
+  a.first = F->args.first;
+  a.item_entry = F->args.item_entry;
+  a.num_entries = F->args.num_entries;
+  a.cloud_blocks = F->args.cloud_blocks;
+  return a;
+}
+
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2)))
+void k_cloud_batch(PersistArgs P) {
+  __shared__ int4v s_b[2 * kPersistMaxNkb * 4 * 64];
+  __shared__ uint32_t s_bias[kMfmaRows * kWaves];
+  // one-K-block families: one workgroup block per workgroup, in the
+  // dispatcher's order (XCD-contiguous, as in their own kernels), ahead of
+  // the resident workgroups
+  const IHE_CONSTANT PersistFamily *fams = cptr(P.fams);
+  if (blockIdx.x < P.once_blocks) {
+    const IHE_CONSTANT uint32_t *once_first = cptr(P.once_first);
+    const uint32_t b = xcd_block(blockIdx.x, P.once_blocks);
+    uint32_t f = 0;
+    while (f + 1 < P.num_once && once_first[f + 1] <= b) ++f;
+    const MfmaArgs args = family_args(fams + f);
+    const uint32_t blk = b - once_first[f];
+    switch (fams[f].body) {
+      case 0: return mfma1_body<false, kPersistNtw1, 1>(args, blk, s_bias, threadIdx.x);
+      case 1: return mfma1_body<false, kPersistNtw1, 2>(args, blk, s_bias, threadIdx.x);
+      case 2: return mfma1_body<true, kPersistNtw1, 1>(args, blk, s_bias, threadIdx.x);
+      default: return mfma1_body<true, kPersistNtw1, 2>(args, blk, s_bias, threadIdx.x);
+    }
+  }
+  // items k and k + 1, and the range taken from. The next item is asked for
+  // as an item starts: its counter value returns behind this item's loads
+  // (the memory counter is in order, stores included), where asking at the
+  // end would wait for all of the item's stores. That value is the one
+  // vector register kept across the bodies; the rest is derived per body
+  // (tid()), as a value hoisted across them would be spilled, and its reloads
+  // too would wait for every store in flight.
+  __shared__ uint32_t s_item[2], s_victim;
+  const uint32_t xcd = blockIdx.x % 8;
+  const bool wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x / 64) == 0;
+  if (wave0) {
+    uint32_t victim = xcd, got = 0;
+    if (threadIdx.x == 0) got = atomicAdd(&P.queue[victim], 1u);
+    const uint32_t item = resolve_item(P, xcd, victim, got);
+    if (threadIdx.x == 0) {
+      s_item[0] = item;
+      s_victim = victim;
+    }
+  }
+  const IHE_CONSTANT uint32_t *fam_first = cptr(P.fam_first);
+  const uint32_t var0 = fam_first[P.num_fams];
+  for (uint32_t k = 0;; k ^= 1) {
+    // the barrier also ends the previous item's reads of LDS
+    __syncthreads();
+    const uint32_t item = __builtin_amdgcn_readfirstlane(s_item[k]);
+    if (item >= P.total) break;
+    uint32_t victim = __builtin_amdgcn_readfirstlane(s_victim), got = 0;
+    if (threadIdx.x == 0) got = atomicAdd(&P.queue[victim], 1u);
+    if (item >= var0) {
+      var_block(P.var, item - var0, tid());
+    } else {
+      uint32_t f = 0;
+      while (f + 1 < P.num_fams && fam_first[f + 1] <= item) ++f;
+      const MfmaArgs args = family_args(fams + P.num_once + f);
+      const uint32_t body = fams[P.num_once + f].body;
+#pragma unroll 1
+      for (uint32_t g = 0; g < kPersistBlocks; ++g) {
+        const uint32_t blk = (item - fam_first[f]) * kPersistBlocks + g;
+        if (blk >= args.cloud_blocks) break;
+        if (g) __syncthreads();
+        switch (body) {
+          case 4: mfmaw_body<false, kPersistNtwW, 2>(args, blk, as_planes<2>(s_b), s_bias, tid()); break;
+          case 5: mfmaw_body<false, kPersistNtwW, 4>(args, blk, as_planes<4>(s_b), s_bias, tid()); break;
+          case 6: mfmaw_body<false, kPersistNtwW6, 6>(args, blk, as_planes<6>(s_b), s_bias, tid()); break;
+          case 7: mfmaw_body<true, kPersistNtwW, 2>(args, blk, as_planes<2>(s_b), s_bias, tid()); break;
+          case 8: mfmaw_body<true, kPersistNtwW, 4>(args, blk, as_planes<4>(s_b), s_bias, tid()); break;
+          default: mfmaw_body<true, kPersistNtwW6, 6>(args, blk, as_planes<6>(s_b), s_bias, tid()); break;
+        }
+      }
+    }
+    if (wave0) {
+      const uint32_t next = resolve_item(P, xcd, victim, got);
+      if (threadIdx.x == 0) {
+        s_item[k ^ 1] = next;
+        s_victim = victim;
+      }
+    }
+  }
+  // every resident workgroup has found no item left: the last one to get here
+  // clears the counters for the next launch (no memset node in a captured graph)
+  if (threadIdx.x == 0 && atomicAdd(&P.queue[8], 1u) == gridDim.x - P.once_blocks - 1)
+    for (uint32_t i = 0; i <= 8; ++i) atomicExch(&P.queue[i], 0u);
+}
+
 // ---------------------------------------------------------------------------
 // Output variances, 256 outputs per block.
 __device__ __forceinline__ uint32_t div_r(uint32_t c, uint32_t R, uint64_t rmagic) {
@@ -572,7 +732,7 @@ __device__ __forceinline__ uint32_t div_r(uint32_t c, uint32_t R, uint64_t rmagi
 // add, and the adds then run in entry order (the order of the model's row).
 // Block `chunk` of the launch's variance work; run by k_cloud_var and by the
 // cloud kernels' trailing blocks.
-__device__ __forceinline__ void var_block(const VarArgs &A, uint32_t chunk) {
+__device__ __forceinline__ void var_block(const VarArgs &A, uint32_t chunk, uint32_t t) {
   VarEntry E = A.single;
   if (A.entries) {
     uint32_t lo = 0, hi = A.num_entries;
@@ -584,7 +744,7 @@ __device__ __forceinline__ void var_block(const VarArgs &A, uint32_t chunk) {
     E = A.entries[lo];
     chunk -= A.first[lo];
   }
-  const uint32_t blk = chunk * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t blk = chunk * 4 + t / 64, lane = t % 64;
   const uint32_t o = blk * 64 + lane;
   if (blk * 64 >= E.num_out) return;
   constexpr int kU = 8;
@@ -691,13 +851,40 @@ void launch_rot(hipStream_t s, const MfmaShape &sh, dim3 grid, const MfmaArgs &A
 
 void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const MfmaEntry &single,
                  const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
-                 uint32_t num_items, const VarArgs &var) {
+                 uint32_t num_items, const VarArgs &var, const uint32_t *item_entry) {
   if (num_items == 0) return launch_var(stream, var);
   const uint32_t cloud_blocks = num_items * mfma_wg_per_tile(shape);
-  const MfmaArgs A = {single, entries, first, num_entries, cloud_blocks, var};
+  const MfmaArgs A = {single, entries, first, item_entry, num_entries, cloud_blocks, var};
   const dim3 grid(cloud_blocks + var.num_blocks);
   if (rot) launch_rot<true>(stream, shape, grid, A);
   else launch_rot<false>(stream, shape, grid, A);
+}
+
+int persist_body(const MfmaShape &shape, bool rot) {
+  static_assert(kPersistNtw1 > 0 && kPersistNtwW > 0 && kPersistNtwW6 > 0, "column tiles");
+  if (shape.nkb == 1 && (shape.rb == 1 || shape.rb == 2)) return (rot ? 2 : 0) + shape.rb - 1;
+  if (shape.rb == kMfmaWideRb && shape.nkb >= 1 && shape.nkb <= (int)kPersistMaxNkb)
+    return (rot ? 7 : 4) + (shape.nkb + 1) / 2 - 1;
+  return -1;
+}
+
+uint32_t persist_wg_per_tile(const MfmaShape &shape) {
+  MfmaShape s = shape;
+  s.ntw = shape.rb != kMfmaWideRb ? kPersistNtw1 : shape.nkb > 4 ? kPersistNtwW6 : kPersistNtwW;
+  return mfma_wg_per_tile(s);
+}
+
+int persist_grid(int device, uint32_t &grid) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      cus <= 0)
+    return fail(IHE_ERR_HIP, "CU count query failed");
+  grid = 8 * ((2u * (uint32_t)cus + 7) / 8);  // two workgroups per CU, a multiple of the 8 XCDs
+  return IHE_OK;
+}
+
+void launch_persistent(hipStream_t stream, const PersistArgs &P, uint32_t grid) {
+  hipLaunchKernelGGL(k_cloud_batch, dim3(P.once_blocks + grid), dim3(64 * kWaves), 0, stream, P);
 }
 
 void launch_var(hipStream_t stream, const VarArgs &var) {

@@ -10,12 +10,16 @@
 // device side: every chunk takes one slot (a context with its own stream,
 // default priority) and one block of a device arena sized by the HBM budget,
 // and runs its whole pipeline there (H2D of its genotypes, encrypt, cloud on
-// the resident band, decrypt, D2H of its scores). Chunks are
-// admitted in arrival order as soon as a slot and arena space are free, so
-// requests that arrive while others run join the device at once and their
-// stages overlap across streams. A chunk's result depends only on its request
-// (seed + chunk index selects the random streams), so it is the same alone
-// and in the mix.
+// the resident band, decrypt, D2H of its scores). Chunks are admitted in
+// arrival order as soon as a slot and arena space are free, so requests that
+// arrive while others run join the device at once and their stages overlap
+// across streams. The scores (4 bytes per output and sample, against about a
+// thousandth of that in device time) cross the link one chunk at a time in
+// admission order: the worker enqueues a chunk's D2H copy, on the chunk's own
+// stream, once its decrypt has ended and the previous chunk's copy has, so
+// transfers run back to back, never several at once, while later chunks
+// compute. A chunk's result depends only on its request (seed + chunk index
+// selects the random streams), so it is the same alone and in the mix.
 //
 // The worker holds the server's mutex only to update the shared state
 // (requests, queue, arena, slot ownership): it polls the slots' events and
@@ -130,14 +134,18 @@ ChunkPlan plan_chunk(const ModelRec &m, uint32_t ns) {
 }
 
 // A slot is busy from the moment a chunk is assigned to it until the worker
-// has seen the chunk's event complete. Only the worker changes a slot, under
-// the mutex (get_stats reads `busy` under it too); the worker reads its slots
-// without the mutex.
+// has seen the chunk's copy complete. Only the worker changes a slot: `busy`
+// (and the arena block) under the mutex, which get_stats reads it under too;
+// the rest, which no other thread reads, without it.
 struct Slot {
   ihe_ctx *ctx = nullptr;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ready = nullptr;  // the chunk's decrypt has ended
+  hipEvent_t ev = nullptr;     // the chunk's scores are on the host
   bool busy = false;
-  bool enqueued = false;  // the chunk's work and event are on the stream
+  bool copying = false;   // its D2H copy and `ev` are on the stream
+  void *dst = nullptr;    // the copy: host destination, device source, bytes
+  const void *src = nullptr;
+  size_t out_bytes = 0;
   uint64_t req = 0;
   size_t off = 0, bytes = 0;
   ChunkPlan plan{};
@@ -152,6 +160,9 @@ struct ihe_server {
   void *arena_base = nullptr;
   Arena arena;
   std::vector<Slot> slots;
+  // slots whose device work is enqueued, in admission order; the first is the
+  // one whose copy runs or runs next (the worker's own: no mutex)
+  std::deque<size_t> order;
   std::vector<ModelRec> models;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -182,17 +193,47 @@ struct ihe_server {
     if (int rc = ihe_cloud_score_layout(s.ctx, m.band, &p.L, p.num_in, d_in, nullptr, d_out, nullptr))
       return rc;
     if (int rc = ihe_decrypt(s.ctx, s.key, ns, d_out, m.num_out, d_dec)) return rc;
-    IHE_HIP(hipMemcpyAsync(r.out + (size_t)m.num_out * col0, d_dec, (size_t)m.num_out * ns * 4,
-                           hipMemcpyDeviceToHost, st));
-    IHE_HIP(hipEventRecord(s.ev, st));
+    IHE_HIP(hipEventRecord(s.ready, st));
+    s.dst = r.out + (size_t)m.num_out * col0;
+    s.src = d_dec;
+    s.out_bytes = (size_t)m.num_out * ns * 4;
     return IHE_OK;
+  }
+
+  // Walks the copy order: starts the first chunk's D2H once its decrypt has
+  // ended, and when that copy has ended, reports the chunk and goes on to the
+  // next. Returns the chunks that ended (slot, status).
+  void advance_copies(std::vector<std::pair<size_t, hipError_t>> &ended) {
+    while (!order.empty()) {
+      const size_t i = order.front();
+      Slot &s = slots[i];
+      if (!s.copying) {
+        hipError_t q = hipEventQuery(s.ready);
+        if (q == hipErrorNotReady) return;
+        if (q == hipSuccess && make_current(s.ctx) != IHE_OK) q = hipErrorInvalidDevice;
+        if (q == hipSuccess)
+          q = hipMemcpyAsync(s.dst, s.src, s.out_bytes, hipMemcpyDeviceToHost, s.ctx->stream);
+        if (q == hipSuccess) q = hipEventRecord(s.ev, s.ctx->stream);
+        if (q != hipSuccess) {
+          (void)hipStreamSynchronize(s.ctx->stream);
+          ended.push_back({i, q});
+          order.pop_front();
+          continue;
+        }
+        s.copying = true;
+      }
+      const hipError_t q = hipEventQuery(s.ev);
+      if (q == hipErrorNotReady) return;
+      ended.push_back({i, q});
+      order.pop_front();
+    }
   }
 
   // The chunk on slot `s` has ended (mutex held).
   void finish_chunk(Slot &s, int rc, const std::string &err) {
     arena.release(s.off, s.bytes);
     s.busy = false;
-    s.enqueued = false;
+    s.copying = false;
     auto it = reqs.find(s.req);
     if (it == reqs.end()) return;
     Request &R = it->second;
@@ -245,7 +286,6 @@ struct ihe_server {
       if (!arena.alloc(p.bytes, off)) break;  // wait for memory, in arrival order
       Slot &s = slots[free_slot];
       s.busy = true;
-      s.enqueued = false;
       s.req = id;
       s.off = off;
       s.bytes = p.bytes;
@@ -261,16 +301,13 @@ struct ihe_server {
     std::vector<Issue> todo;
     std::vector<std::pair<size_t, hipError_t>> ended;
     for (;;) {
-      // slots whose chunk has ended (event queries without the mutex)
+      // copies to start and chunks that have ended (event queries without the mutex)
       ended.clear();
-      for (size_t i = 0; i < slots.size(); ++i) {
-        Slot &s = slots[i];
-        if (!s.busy || !s.enqueued) continue;
-        const hipError_t q = hipEventQuery(s.ev);
-        if (q == hipErrorNotReady) continue;
+      advance_copies(ended);
+      for (const auto &e : ended) {
+        Slot &s = slots[e.first];
         if (s.key) ihe_key_destroy(s.key);
         s.key = nullptr;
-        ended.push_back({i, q});
       }
       todo.clear();
       bool busy = false, progress = !ended.empty();
@@ -300,7 +337,7 @@ struct ihe_server {
           std::lock_guard<std::mutex> lk(mu);
           finish_chunk(s, rc, err);
         } else {
-          s.enqueued = true;
+          order.push_back(t.slot);
         }
       }
     }
@@ -333,11 +370,13 @@ int ihe_server_create(ihe_ctx *ctx, const ihe_server_config *cfg, ihe_server **o
   s->slots.resize(cfg->num_slots);
   for (Slot &sl : s->slots) {
     int rc = ihe_ctx_create(ctx->device, &sl.ctx);
-    if (!rc && hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess)
+    if (!rc && (hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming) != hipSuccess))
       rc = fail(IHE_ERR_HIP, "event creation failed");
     if (rc) {
       for (Slot &o : s->slots) {
         if (o.ev) (void)hipEventDestroy(o.ev);
+        if (o.ready) (void)hipEventDestroy(o.ready);
         if (o.ctx) ihe_ctx_destroy(o.ctx);
       }
       (void)ihe_free(ctx, s->arena_base);
@@ -440,6 +479,7 @@ int ihe_server_destroy(ihe_server *s) {
   if (s->worker.joinable()) s->worker.join();
   for (Slot &sl : s->slots) {
     if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.ready) (void)hipEventDestroy(sl.ready);
     if (sl.ctx) ihe_ctx_destroy(sl.ctx);
   }
   (void)ihe_free(s->ctx, s->arena_base);

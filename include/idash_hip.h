@@ -198,9 +198,13 @@ int ihe_cloud_score_layout(ihe_ctx *ctx, const ihe_band *band, const ihe_layout 
 /* Many cohorts in one step, in place of one cloud_compute_score call per
  * cohort (eval/cloud.cpp:17): each cohort has its own packing layout, input
  * ciphertexts and resident MFMA band (bands may be shared). The work is one
- * launch per MFMA kernel family plus one variance launch, and
- * ihe_batch_capture records them into a hipGraph so that ihe_batch_run is a
- * single graph launch. Results equal ihe_cloud_score_layout per cohort. */
+ * persistent kernel whose workgroups take every cohort's tiles and variance
+ * blocks from device counters (bands tiled 32 or 64 rows x 1 K-block, or 128
+ * rows x up to 6 K-blocks); a batch with other tilings, or built with
+ * IHE_BATCH_PERSISTENT=0, runs one launch per kernel family instead.
+ * ihe_batch_capture records the work into a hipGraph so that ihe_batch_run is
+ * a single graph launch. Results equal ihe_cloud_score_layout per cohort.
+ * Runs of one batch are stream-ordered; two batches may run concurrently. */
 typedef struct ihe_batch ihe_batch;
 typedef struct {
   const ihe_band *band;    /* MFMA band */
@@ -213,6 +217,7 @@ typedef struct {
 } ihe_cohort;
 typedef struct {
   uint32_t num_cohorts, num_launches, num_tiles, captured;
+  uint32_t persistent; /* 1: one persistent kernel runs every family and the variances */
 } ihe_batch_info;
 /* The batch keeps device pointers to the cohorts' buffers and bands: both must
  * outlive it (ihe_band_destroy fails with IHE_ERR_BUSY while a batch holds the

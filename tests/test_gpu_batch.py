@@ -1,8 +1,8 @@
 """Batched cloud step (ihe_batch_*) and the kernel autotuner
 (ihe_band_create_tuned): cohorts of uneven sample counts on resident model
-bands of several neighbour counts, in one launch per kernel family and as a
-captured hipGraph; every cohort bit-exact against its own per-cohort launch
-and, on small sizes, against the oracle."""
+bands of several neighbour counts, in one persistent launch (or one launch per
+kernel family) and as a captured hipGraph; every cohort bit-exact against its
+own per-cohort launch and, on small sizes, against the oracle."""
 import numpy as np
 import pytest
 
@@ -59,9 +59,16 @@ def setup(engine):
         b.free()
 
 
-def make_cohorts(engine, models, bands, sample_counts, with_var=True):
+def persistent_capable(info):
+    """Tilings the persistent batch kernel runs: one K-block over 32 or 64
+    rows, or 128 rows over at most 6 K-blocks."""
+    return (info["tile_rows"] in (32, 64) and info["tile_kblocks"] == 1) or \
+        (info["tile_rows"] == 128 and info["tile_kblocks"] <= 6)
+
+
+def make_cohorts(engine, models, bands, sample_counts, with_var=True, keys=None):
     rng = np.random.default_rng(7)
-    keys = list(models)
+    keys = keys or list(models)
     cohorts, host = [], []
     for i, ns in enumerate(sample_counts):
         key = keys[i % len(keys)]
@@ -78,10 +85,22 @@ def make_cohorts(engine, models, bands, sample_counts, with_var=True):
     return cohorts, host
 
 
-def test_batch_matches_per_cohort_launches_and_oracle(engine, setup):
+@pytest.mark.parametrize("mode", ["persistent", "families"])
+def test_batch_matches_per_cohort_launches_and_oracle(engine, setup, mode, monkeypatch):
+    """Persistent: cohorts on every band whose tiling the persistent kernel
+    runs (both row-block kinds, rotated and not), one launch. Families: every
+    band, including per-wave multi-K-block tilings, one launch per family."""
     models, bands = setup
     counts = [1024, 700, 341, 200, 128, 100, 64, 33, 16, 1024, 512, 77]
-    cohorts, host = make_cohorts(engine, models, bands, counts)
+    keys = None
+    if mode == "families":
+        monkeypatch.setenv("IHE_BATCH_PERSISTENT", "0")
+    else:
+        monkeypatch.delenv("IHE_BATCH_PERSISTENT", raising=False)
+        keys = [k for k in models if persistent_capable(bands[k].info())]
+        rows = {bands[k].info()["tile_rows"] for k in keys}
+        assert len(keys) >= 4 and 128 in rows and rows & {32, 64}, keys
+    cohorts, host = make_cohorts(engine, models, bands, counts, keys=keys)
     # per-cohort launches
     ref = []
     for band, ns, num_in, d_in, d_var, d_out, d_ov in cohorts:
@@ -92,8 +111,13 @@ def test_batch_matches_per_cohort_launches_and_oracle(engine, setup):
     batch = engine.batch(cohorts)
     info = batch.info()
     assert info["num_cohorts"] == len(counts)
-    assert 2 <= info["num_launches"] <= 1 + 2 * 4 * 3   # families + the variance launch
+    if mode == "persistent":
+        assert info["persistent"] == 1 and info["num_launches"] == 1
+    else:
+        assert info["persistent"] == 0
+        assert 2 <= info["num_launches"] <= 2 * 4 * 3     # one per family
     batch.run()
+    batch.run()                                 # the work counters start at zero again
     for (_, _, _, _, _, d_out, d_ov), (o, v) in zip(cohorts, ref):
         assert (d_out.numpy() == o).all() and (d_ov.numpy() == v).all()
     # the oracle on a few cohorts (including rotated layouts)

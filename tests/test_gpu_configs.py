@@ -155,14 +155,19 @@ def test_small_workload_through_the_executables(tmp_path):
 # ---------------------------------------------------------------- wide
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("kernel", ["auto", "staged"])
+@pytest.mark.parametrize("kernel", ["auto", "4,8", "staged"])
 def test_wide_workload_full_layout(engine, kernel, monkeypatch):
     """n = 50 on the full layout: 150 one-hot columns per output, 242,646
     outputs from 48,552 device-encrypted inputs; the MFMA kernel (the default
-    for wide windows) and the LDS-staged VALU kernel."""
-    monkeypatch.delenv("IHE_CLOUD_KERNEL", raising=False)
-    if kernel != "auto":
+    for wide windows) on its own choice of tile and on 128-row x 8-K-block
+    tiles (two K-blocks of padding on most), and the LDS-staged VALU kernel."""
+    for k in ("IHE_CLOUD_KERNEL", "IHE_CLOUD_SHAPE", "IHE_CLOUD_NTW"):
+        monkeypatch.delenv(k, raising=False)
+    if kernel == "staged":
         monkeypatch.setenv("IHE_CLOUD_KERNEL", kernel)
+    elif kernel != "auto":
+        monkeypatch.setenv("IHE_CLOUD_SHAPE", kernel)
+        monkeypatch.setenv("IHE_CLOUD_NTW", "32")
     c = synth.workload("wide")
     ns, R, RS = c.layout
     assert (c.num_in_cts, c.num_outputs, c.neighbours) == (48552, 242646, 50)
@@ -176,8 +181,10 @@ def test_wide_workload_full_layout(engine, kernel, monkeypatch):
     band = engine.band(ns, c.num_in_cts, rp, ib, w)
     info = band.info()
     assert info["max_window"] == 150
-    if kernel == "auto":
-        assert info["variant"] == _abi.VARIANT_MFMA and info["tile_rows"] == 128
+    if kernel != "staged":
+        assert info["variant"] == _abi.VARIANT_MFMA
+    if kernel == "4,8":
+        assert (info["tile_rows"], info["tile_kblocks"]) == (32 * int(kernel[0]), int(kernel[2]))
     d_out = engine.empty((c.num_outputs, 2, 1024), np.int32)
     d_ov = engine.empty((c.num_outputs,), np.float64)
     engine.cloud(band, d_ct, d_var, d_out, d_ov)
@@ -194,17 +201,25 @@ def test_wide_workload_full_layout(engine, kernel, monkeypatch):
 # ---------------------------------------------------------------- batched
 
 @pytest.mark.timeout(900)
-def test_batched_workload_full_layout(engine):
+def test_batched_workload_full_layout(engine, monkeypatch):
     """64 cohorts drawn as the batched workload is: NS ~ U[16, 1024] (R from 1
     to 64), one of 3 model sets, n in {5, 10, 20, 50}, all on the 16,184-tag /
-    80,882-target layout; inputs encrypted on the device with a key per cohort."""
+    80,882-target layout; inputs encrypted on the device with a key per cohort.
+    The bands take the tilings the batched bench tunes to (64 rows x 1 K-block
+    at n = 5, 128 rows x 2, 4, 6 K-blocks above), so the batch is one
+    persistent launch."""
     base = synth.workload("full")
     rng = np.random.default_rng(4)
     models = synth.model_sets(base, (5, 10, 20, 50), 3, 4)
     specs = [(int(rng.integers(16, 1025)), int(rng.integers(0, 3)),
               int(rng.choice([5, 10, 20, 50]))) for _ in range(64)]
     csrs = {k: models[k].model_csr() for k in sorted({(s, n) for _, s, n in specs})}
-    bands = {k: engine.band(1024, base.num_in_cts, *csr) for k, csr in csrs.items()}
+    shapes = {5: "2,1", 10: "4,2", 20: "4,4", 50: "4,6"}
+    bands = {}
+    for k, csr in csrs.items():
+        monkeypatch.setenv("IHE_CLOUD_SHAPE", shapes[k[1]])
+        bands[k] = engine.band(1024, base.num_in_cts, *csr)
+    monkeypatch.delenv("IHE_CLOUD_SHAPE")
     cohorts, keys = [], []
     for i, (ns, s, n) in enumerate(specs):
         R = 1024 // ns
@@ -223,6 +238,7 @@ def test_batched_workload_full_layout(engine):
     packings = {1024 // ns for ns, _, _ in specs}
     assert len(packings) >= 8 and 1 in packings and max(packings) >= 16
     batch = engine.batch(cohorts)
+    assert batch.info()["persistent"] == 1 and batch.info()["num_launches"] == 1
     batch.run()
     engine.sync()
     sums = [(engine.checksum(co[5]), engine.checksum(co[6])) for co in cohorts]
