@@ -940,10 +940,12 @@ uint32_t count_tiles(uint32_t num_out, const MfmaShape &shape, const HostBand &H
 }
 
 // Column tiles per wave (per workgroup for the four-row-block kernel):
-// IHE_CLOUD_NTW if it names one the kernel family has, else its default.
-int pick_ntw(int rb) {
+// IHE_CLOUD_NTW if it names one the kernel family has, else its default: 4
+// per wave; 16 per workgroup up to 4 K-blocks, 32 above (n = 50 on the full
+// layout, 128 x 6 tiles: 0.887 ms at 32, 0.911 at 16, 0.951 at 8).
+int pick_ntw(int rb, int nkb) {
   const int v = (int)env_u32("IHE_CLOUD_NTW", 0, 4, 32);
-  if (rb == kMfmaWideRb) return v == 8 || v == 32 ? v : 16;
+  if (rb == kMfmaWideRb) return v == 8 || v == 16 || v == 32 ? v : nkb > 4 ? 32 : 16;
   return v == 8 || v == 16 ? v : 4;
 }
 
@@ -961,18 +963,18 @@ bool pick_mfma_shape(uint32_t num_out, HostBand &H, MfmaShape &shape) {
     if (std::sscanf(e, "%d,%d", &rb, &nkb) == 2 &&
         (rb == 1 || (rb == 2 && nkb == 1) || rb == kMfmaWideRb) && nkb >= 1 &&
         nkb <= (int)kMfmaMaxNkb) {
-      shape = {rb, nkb, pick_ntw(rb)};
+      shape = {rb, nkb, pick_ntw(rb, nkb)};
       return count_tiles(num_out, shape, H) > 0;
     }
   }
   const int kmin = std::max<int>(1, (int)((H.max_window + 31) / 32));
   std::vector<MfmaShape> cands;
-  if (kmin == 1) cands.push_back({2, 1, pick_ntw(2)}), cands.push_back({1, 1, pick_ntw(1)});
+  if (kmin == 1) cands.push_back({2, 1, pick_ntw(2, 1)}), cands.push_back({1, 1, pick_ntw(1, 1)});
   for (int k = std::max(2, kmin); k <= std::min<int>(kmin + 2, (int)kMfmaMaxNkb); ++k)
-    cands.push_back({1, k, pick_ntw(1)});
+    cands.push_back({1, k, pick_ntw(1, k)});
   // the four-row-block kernel runs an even number of K-blocks
   for (int k = kmin + kmin % 2; k <= std::min<int>(kmin + 3, (int)kMfmaMaxNkb); k += 2)
-    cands.push_back({kMfmaWideRb, k, pick_ntw(kMfmaWideRb)});
+    cands.push_back({kMfmaWideRb, k, pick_ntw(kMfmaWideRb, k)});
   double best = 0;
   bool found = false;
   for (const MfmaShape &c : cands) {

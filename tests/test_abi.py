@@ -99,8 +99,8 @@ def test_band_plan_groups_shared_windows(monkeypatch):
     assert (w["variant"], w["tile_rows"], w["tile_kblocks"], w["col_tiles"]) == \
         (_abi.VARIANT_MFMA, 128, 8, 32)
     assert w["num_tiles"] == -(-c.num_outputs // 128)
-    monkeypatch.setenv("IHE_CLOUD_NTW", "4")            # not a column-tile count of that kernel
-    assert band_plan(c.num_samples, c.num_in_cts, *c.model_csr())["col_tiles"] == 16
+    monkeypatch.setenv("IHE_CLOUD_NTW", "4")            # not a column-tile count of that kernel:
+    assert band_plan(c.num_samples, c.num_in_cts, *c.model_csr())["col_tiles"] == 32  # the default
     monkeypatch.delenv("IHE_CLOUD_NTW")
     # one K-block: tiles are cut where the union passes 32 columns
     monkeypatch.setenv("IHE_CLOUD_SHAPE", "4,1")
