@@ -95,8 +95,9 @@ def build_bands(eng, models, keys, num_in, shapes=None):
 
 def forms_batched(eng, base, log_fn, seed=4):
     """64 cohorts, NS ~ U[16, 1024], 3 model sets x neighbours {5, 10, 20, 50} on the
-    full SNP layout: 64 per-cohort launches vs one batch (one launch per kernel family)
-    vs its hipGraph replay; every cohort's output checksum must agree."""
+    full SNP layout: 64 per-cohort launches vs one launch per kernel family vs the one
+    batch launch (k_cloud_batch) vs its hipGraph replay; every cohort's output checksum
+    must agree."""
     rng = np.random.default_rng(seed)
     models = synth.model_sets(base, (5, 10, 20, 50), 3, seed)
     specs = [(int(rng.integers(16, 1025)), int(rng.integers(0, 3)), int(rng.choice([5, 10, 20, 50])))

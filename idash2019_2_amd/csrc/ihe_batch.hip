@@ -163,7 +163,7 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
       vfirst.push_back(vfirst.back() + (band->num_out + 255) / 256);
     }
   }
-  // one launch per kernel family, cohorts in order within it
+  // kernel families (tile shape x rotation), cohorts in order within each
   std::stable_sort(keyed.begin(), keyed.end(), [](const Keyed &a, const Keyed &b) {
     if (a.rot != b.rot) return a.rot < b.rot;
     if (a.shape.nkb != b.shape.nkb) return a.shape.nkb < b.shape.nkb;

@@ -128,7 +128,8 @@ def check_cloud_buffers(band: "Band", num_in: int, d_in, d_in_var, d_out, d_out_
 
 
 class Batch:
-    """Many cohorts' cloud steps as one launch per kernel family (ihe_batch_*)."""
+    """Many cohorts' cloud steps in one kernel launch (ihe_batch_*), or one launch
+    per kernel family when a family has no body in that kernel."""
 
     def __init__(self, engine: "Engine", cohorts):
         self.engine = engine
