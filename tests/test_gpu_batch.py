@@ -204,3 +204,46 @@ def test_a_band_outlives_its_batch(engine, setup):
     assert e.value.code == _abi.IHE_ERR_BUSY
     batch.free()
     band.free()
+
+
+@pytest.mark.parametrize("kinds", ["one-K-block only", "128-row only", "one cohort"])
+@pytest.mark.parametrize("with_var", [True, False])
+def test_one_launch_edge_mixes(engine, setup, kinds, with_var):
+    """The one-launch batch when one of its two parts is empty: only one-K-block
+    families (the resident workgroups get variance blocks or nothing), only
+    128-row families (no leading blocks), or a single cohort; with and without
+    variances. Two runs in a row (the work counters start at zero again), each
+    bit-exact against the per-cohort launches."""
+    models, bands = setup
+    rows = {k: bands[k].info()["tile_rows"] for k in models}
+    kb = {k: bands[k].info()["tile_kblocks"] for k in models}
+    if kinds == "one-K-block only":
+        keys = [k for k in models if rows[k] in (32, 64) and kb[k] == 1]
+        counts = [1024, 341, 100, 16, 1024]
+    elif kinds == "128-row only":
+        keys = [k for k in models if rows[k] == 128 and kb[k] <= 6]
+        counts = [700, 1024, 33, 128]
+    else:
+        keys = [k for k in models if persistent_capable(bands[k].info())][:1]
+        counts = [200]
+    assert keys, kinds
+    cohorts, _ = make_cohorts(engine, models, bands, counts, with_var=with_var, keys=keys)
+    ref = []
+    for band, ns, num_in, d_in, d_var, d_out, d_ov in cohorts:
+        engine.cloud_layout(band, ns, num_in, d_in, d_var if with_var else None, d_out, d_ov)
+        ref.append((d_out.numpy(), d_ov.numpy() if with_var else None))
+    batch = engine.batch(cohorts)
+    assert batch.info()["persistent"] == 1 and batch.info()["num_launches"] == 1
+    for _ in range(2):
+        for c in cohorts:
+            c[5].fill_zero()
+            if with_var:
+                c[6].fill_zero()
+        engine.sync()
+        batch.run()
+        engine.sync()
+        for c, (o, v) in zip(cohorts, ref):
+            assert (c[5].numpy() == o).all()
+            if with_var:
+                assert (c[6].numpy() == v).all()
+    batch.free()
