@@ -451,8 +451,14 @@ int ihe_server_wait(ihe_server *s, uint64_t ticket, double *latency_ms) {
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->reqs.find(ticket);
   if (it == s->reqs.end()) return fail(IHE_ERR_ARG, "unknown ticket");
-  s->cv_done.wait(lk, [&] { return s->reqs.at(ticket).done; });
-  return collect(s, s->reqs.find(ticket), latency_ms);
+  // another thread's poll or wait may collect the ticket meanwhile
+  s->cv_done.wait(lk, [&] {
+    auto f = s->reqs.find(ticket);
+    return f == s->reqs.end() || f->second.done;
+  });
+  it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return fail(IHE_ERR_ARG, "ticket collected by another call");
+  return collect(s, it, latency_ms);
 }
 
 int ihe_server_get_stats(ihe_server *s, ihe_server_stats *st) {

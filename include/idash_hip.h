@@ -271,8 +271,8 @@ int ihe_server_add_model(ihe_server *server, const ihe_band *band, uint32_t num_
 /* The request's host buffers must stay valid until it completes. */
 int ihe_server_submit(ihe_server *server, const ihe_request *req, uint64_t *ticket);
 /* Completed requests report their latency (arrival, or submit when the
- * request gives none, to scores on the host) and are forgotten; one waiter per
- * ticket. */
+ * request gives none, to scores on the host) and are forgotten: one waiter per
+ * ticket (any other poll or wait on it then fails with IHE_ERR_ARG). */
 int ihe_server_poll(ihe_server *server, uint64_t ticket, int *done, double *latency_ms);
 int ihe_server_wait(ihe_server *server, uint64_t ticket, double *latency_ms);
 int ihe_server_get_stats(ihe_server *server, ihe_server_stats *stats);

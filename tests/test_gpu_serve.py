@@ -162,3 +162,35 @@ def test_a_band_outlives_its_server(engine, models):
     assert e.value.code == _abi.IHE_ERR_BUSY
     srv.close()
     b2.free()
+
+
+def test_two_waiters_on_one_ticket(engine, models):
+    """Two threads wait on one ticket: one collects it, the other gets
+    IHE_ERR_ARG (not a crash), whichever of them the worker wakes first."""
+    import threading
+    c, band = models[5]
+    srv = engine.server(4 * GIB, num_slots=2)
+    m = srv.add_model(band, c.num_tags)
+    geno = np.zeros((c.num_tags, 1024), np.int8)
+    out = np.zeros((c.num_outputs, 1024), np.float32)
+    t = srv.submit(m, geno, np.zeros(1024, np.int32), 1, out)
+    got = []
+
+    def waiter():
+        try:
+            got.append(("ok", srv.wait(t)))
+        except IheError as e:
+            got.append(("err", e.code))
+
+    th = [threading.Thread(target=waiter) for _ in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=60)
+    assert not any(x.is_alive() for x in th)
+    assert sorted(k for k, _ in got) == ["err", "ok"]
+    assert [v for k, v in got if k == "err"] == [_abi.IHE_ERR_ARG]
+    with pytest.raises(IheError) as e:
+        srv.poll(t)
+    assert e.value.code == _abi.IHE_ERR_ARG
+    srv.close()
