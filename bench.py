@@ -330,6 +330,9 @@ def cli_pipeline(workload, log_fn, workdir=None):
         vals = {k.strip(" ."): float(v) for k, v in re.findall(r"^(.*?):\s*([0-9.e+-]+)\s*$",
                                                                    r.stdout, re.M)}
         vals["process wall time (s)"] = round(time.time() - t, 3)
+        for ln in r.stderr.splitlines():  # IHE_HOST_PHASES=1: the host functions' phases
+            if " phase " in ln:
+                log_fn(f"cli: {name}: {ln.strip()}")
         log_fn(f"cli: {name} {' '.join(args)}: {vals}")
         return vals
 

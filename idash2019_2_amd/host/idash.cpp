@@ -6,23 +6,30 @@
 // include/idash_hip.h; a missing GPU is an error, never a CPU fallback.
 #include "idash.h"
 
+#include <fcntl.h>
+#include <sys/mman.h>
 #include <sys/resource.h>
 #include <sys/stat.h>
 #include <unistd.h>
 #include <sys/time.h>
 
 #include <algorithm>
+#include <atomic>
 #include <charconv>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
 #include <fstream>
+#include <memory>
+#include <future>
 #include <random>
 #include <sstream>
 #include <thread>
 
+#include "gfmt.h"
 #include "idash_hip.h"
 #include "parse_vw.h"
 
@@ -114,6 +121,26 @@ struct DeviceKey {
 };
 
 ihe_layout layout_of(const IdashParams &p) { return {p.NUM_SAMPLES, p.NUM_REGIONS, p.REGION_SIZE}; }
+
+// Wall-clock phases of a host function, printed to stderr when IHE_HOST_PHASES
+// is set (the executables' stdout printouts stay as the reference's).
+struct Phases {
+  const char *fn;
+  bool on = std::getenv("IHE_HOST_PHASES") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  explicit Phases(const char *f) : fn(f) {}
+  void mark(const char *what) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "%s phase %-28s %9.3f ms\n", fn, what,
+                 std::chrono::duration<double, std::milli>(now - t).count());
+    t = now;
+  }
+  // a figure summed over a loop (not a span since the last mark)
+  void note(const char *what, double ms) const {
+    if (on) std::fprintf(stderr, "%s phase %-28s %9.3f ms (summed)\n", fn, what, ms);
+  }
+};
 
 uint64_t fresh_seed() {
   if (const char *s = std::getenv("IHE_SEED")) {
@@ -416,6 +443,7 @@ void save_cache(const string &file, uint64_t sig, const std::vector<HrFile> &fil
 // are parsed in parallel, and a cache in the model directory replaces the
 // parsing while every file keeps its size and modification time.
 void read_model(Model &model, const IdashParams &params, const string &path) {
+  Phases ph("read_model");
   std::vector<HrFile> files;
   files.reserve(3 * params.out_features_index.size());
   for (const auto &e : params.out_features_index)
@@ -430,6 +458,7 @@ void read_model(Model &model, const IdashParams &params, const string &path) {
   const string cache = path + "/.ihe_model_cache";
   bool all_present = true;
   for (char c : present) all_present = all_present && c;
+  ph.mark("stat of the .hr files");
   if (!(all_present && load_cache(cache, sig, files))) {
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t i = 0; i < n; ++i) {
@@ -446,11 +475,16 @@ void read_model(Model &model, const IdashParams &params, const string &path) {
     }
     save_cache(cache, sig, files);
   }
-  for (const HrFile &h : files) {
-    std::unordered_map<FeatBigIndex, int32_t> &m = model.model[h.out_bidx] =
-        std::unordered_map<FeatBigIndex, int32_t>();
-    for (const auto &c : h.entries) m[c.first] = c.second;
-  }
+  ph.mark("model cache or .hr parse");
+  // each output's map is filled by one thread, in its file's order (the same
+  // insertions as a sequential reader makes, so the same iteration order),
+  // then moved into the model in file order
+  std::vector<std::unordered_map<FeatBigIndex, int32_t>> rows(files.size());
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; ++i)
+    for (const auto &c : files[i].entries) rows[i][c.first] = c.second;
+  for (size_t i = 0; i < files.size(); ++i) model.model[files[i].out_bidx] = std::move(rows[i]);
+  ph.mark("model maps");
 }
 
 // ---------------------------------------------------------------------------
@@ -578,79 +612,114 @@ IdashKey *keygen_ph2(IdashParams *idashParams) {
 
 namespace {
 
-class RecordWriter {
- public:
-  explicit RecordWriter(const string &filename) : f_(std::fopen(filename.c_str(), "wb")), buf_(kBuf) {}
-  ~RecordWriter() {
-    flush();
-    if (f_) std::fclose(f_);
-  }
-  bool ok() const { return f_ != nullptr; }
-  void raw(const void *p, size_t n) {
-    if (used_ + n > buf_.size()) flush();
-    if (n > buf_.size()) {
-      REQUIRE_DRAMATICALLY(std::fwrite(p, 1, n, f_) == n, "write failed");
-      return;
+constexpr size_t kRecordBytes = 4 + kSampleStreamBytes;  // index + sample stream: 8212 bytes
+
+// Writes the count and one record per (index, sample), in the given order
+// (with the collection's framing word). Records have a fixed size, so threads
+// pack contiguous ranges of them into buffers of their own and write each at
+// its file offset: the file is the one a sequential writer makes.
+void write_records(const string &filename,
+                   const std::vector<std::pair<uint32_t, const TLweSample *>> &recs,
+                   int32_t stream_word, const char *open_error) {
+  const int fd = ::open(filename.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+  REQUIRE_DRAMATICALLY(fd >= 0, open_error);
+  auto write_at = [fd](const char *p, size_t n, off_t off) {
+    while (n) {
+      const ssize_t w = ::pwrite(fd, p, n, off);
+      if (w <= 0) return false;
+      p += w, n -= (size_t)w, off += w;
     }
-    std::memcpy(buf_.data() + used_, p, n);
-    used_ += n;
+    return true;
+  };
+  const uint64_t n = recs.size();
+  bool ok = write_at(reinterpret_cast<const char *>(&n), 8, 0);
+  constexpr size_t kPer = 1024;  // records per buffer (8.4 MB)
+  const int64_t blocks = (int64_t)((n + kPer - 1) / kPer);
+#pragma omp parallel reduction(&& : ok)
+  {
+    std::vector<char> buf(kPer * kRecordBytes);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t blk = 0; blk < blocks; ++blk) {
+      const size_t r0 = (size_t)blk * kPer, r1 = std::min<size_t>(n, r0 + kPer);
+      char *p = buf.data();
+      for (size_t r = r0; r < r1; ++r, p += kRecordBytes) {
+        const TLweSample *s = recs[r].second;
+        std::memcpy(p, &recs[r].first, 4);
+        std::memcpy(p + 4, &stream_word, 4);
+        std::memcpy(p + 8, s->a, IdashParams::N * 4);
+        std::memcpy(p + 8 + IdashParams::N * 4, s->b, IdashParams::N * 4);
+        std::memcpy(p + 8 + kCtBytes, s->current_variance, 8);
+      }
+      ok = ok && write_at(buf.data(), (r1 - r0) * kRecordBytes, (off_t)(8 + r0 * kRecordBytes));
+    }
   }
-  void record(uint32_t index, const TLweSample *s, int32_t stream_word) {
-    raw(&index, 4);
-    raw(&stream_word, 4);
-    raw(s->a, IdashParams::N * 4);
-    raw(s->b, IdashParams::N * 4);
-    raw(s->current_variance, 8);
-  }
-  void flush() {
-    if (used_) REQUIRE_DRAMATICALLY(std::fwrite(buf_.data(), 1, used_, f_) == used_, "write failed");
-    used_ = 0;
-  }
+  ok = ::close(fd) == 0 && ok;
+  REQUIRE_DRAMATICALLY(ok, "write failed");
+}
 
- private:
-  static constexpr size_t kBuf = 64u << 20;
-  FILE *f_;
-  std::vector<char> buf_;
-  size_t used_ = 0;
-};
-
-// Reads `count` records into a dense block of `slots` samples (slot = index).
-// Slots no record fills are zeroed. Every record of a file must open its
-// sample stream with the same framing word, which the block keeps.
+// Reads the records into a dense block of `slots` samples (slot = index) and
+// binds them in `map`; slots no record fills are zeroed. Every record must
+// open its sample stream with the same framing word, which the block keeps.
+// The file is mapped a window of records at a time: the window's headers are
+// checked in file order (the errors of a sequential reader), then its
+// payloads are copied to their slots by several threads.
 template <typename Map>
 std::shared_ptr<SampleBlock> read_records(const string &filename, size_t slots, Map &map,
                                           const char *what) {
-  FILE *f = std::fopen(filename.c_str(), "rb");
-  REQUIRE_DRAMATICALLY(f, "Cannot open " << what << " file for read");
+  Phases ph("read_records");
+  const int fd = ::open(filename.c_str(), O_RDONLY);
+  REQUIRE_DRAMATICALLY(fd >= 0, "Cannot open " << what << " file for read");
+  struct stat st;
+  REQUIRE_DRAMATICALLY(::fstat(fd, &st) == 0, "Cannot open " << what << " file for read");
+  const size_t size = (size_t)st.st_size;
   uint64_t count = 0;
-  REQUIRE_DRAMATICALLY(std::fread(&count, 8, 1, f) == 1, "truncated " << what << " file");
+  REQUIRE_DRAMATICALLY(size >= 8 && ::pread(fd, &count, 8, 0) == 8, "truncated " << what << " file");
+  REQUIRE_DRAMATICALLY(count <= (size - 8) / kRecordBytes, "truncated " << what << " file");
   auto block = std::make_shared<SampleBlock>(slots);
-  std::vector<char> rec(4 + kSampleStreamBytes);
   std::vector<char> filled(slots, 0);
-  for (uint64_t i = 0; i < count; ++i) {
-    REQUIRE_DRAMATICALLY(std::fread(rec.data(), rec.size(), 1, f) == 1, "truncated " << what << " file");
-    uint32_t index;
-    int32_t word;
-    std::memcpy(&index, rec.data(), 4);
-    std::memcpy(&word, rec.data() + 4, 4);
-    REQUIRE_DRAMATICALLY(index < slots, what << " index out of range");
-    if (i == 0) block->stream_word = word;
-    REQUIRE_DRAMATICALLY(word == block->stream_word,
-                         what << " records open with different framing words");
-    TLweSample *s = block->sample(index);
-    REQUIRE_DRAMATICALLY(map.emplace(index, s).second, "duplicate " << what << " index");
-    filled[index] = 1;
-    std::memcpy(s->a, rec.data() + 8, IdashParams::N * 4);
-    std::memcpy(s->b, rec.data() + 8 + IdashParams::N * 4, IdashParams::N * 4);
-    std::memcpy(s->current_variance, rec.data() + 8 + kCtBytes, 8);
+  const size_t page = (size_t)::sysconf(_SC_PAGESIZE);
+  constexpr uint64_t kWindow = 8192;  // records per mapping (67 MB)
+  std::vector<uint32_t> index(kWindow);
+  for (uint64_t r0 = 0; r0 < count; r0 += kWindow) {
+    const uint64_t r1 = std::min(count, r0 + kWindow);
+    const size_t off = 8 + r0 * kRecordBytes, moff = off & ~(page - 1);
+    const size_t len = 8 + r1 * kRecordBytes - moff;
+    void *m = ::mmap(nullptr, len, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, (off_t)moff);
+    REQUIRE_DRAMATICALLY(m != MAP_FAILED, "Cannot read " << what << " file");
+    const char *recs = static_cast<const char *>(m) + (off - moff);
+    for (uint64_t i = r0; i < r1; ++i) {
+      const char *rec = recs + (i - r0) * kRecordBytes;
+      uint32_t ix;
+      int32_t word;
+      std::memcpy(&ix, rec, 4);
+      std::memcpy(&word, rec + 4, 4);
+      REQUIRE_DRAMATICALLY(ix < slots, what << " index out of range");
+      if (i == 0) block->stream_word = word;
+      REQUIRE_DRAMATICALLY(word == block->stream_word,
+                           what << " records open with different framing words");
+      REQUIRE_DRAMATICALLY(map.emplace(ix, block->sample(ix)).second, "duplicate " << what << " index");
+      filled[ix] = 1;
+      index[i - r0] = ix;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)(r1 - r0); ++i) {
+      const char *rec = recs + (size_t)i * kRecordBytes;
+      TLweSample *s = block->sample(index[i]);
+      std::memcpy(s->a, rec + 8, IdashParams::N * 4);
+      std::memcpy(s->b, rec + 8 + IdashParams::N * 4, IdashParams::N * 4);
+      std::memcpy(s->current_variance, rec + 8 + kCtBytes, 8);
+    }
+    ::munmap(m, len);
   }
-  std::fclose(f);
-  for (size_t i = 0; i < slots; ++i)
+  ::close(fd);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)slots; ++i)
     if (!filled[i]) {
       TLweSample *s = block->sample(i);
       std::memset(s->a, 0, kCtBytes);
       *s->current_variance = 0.0;
     }
+  ph.mark(what);
   return block;
 }
 
@@ -668,12 +737,10 @@ size_t num_input_cts(const IdashParams &p) {
 
 void write_encrypted_data(const EncryptedData &encrypted_data, const IdashParams &,
                           const string &filename) {
-  RecordWriter w(filename);
-  REQUIRE_DRAMATICALLY(w.ok(), "Cannot open encrypted data file for write");
-  const uint64_t n = encrypted_data.enc_data.size();
-  w.raw(&n, 8);
-  const int32_t word = stream_word_of(encrypted_data.block);
-  for (const auto &e : encrypted_data.enc_data) w.record(e.first, e.second, word);
+  std::vector<std::pair<uint32_t, const TLweSample *>> recs(encrypted_data.enc_data.begin(),
+                                                            encrypted_data.enc_data.end());
+  write_records(filename, recs, stream_word_of(encrypted_data.block),
+                "Cannot open encrypted data file for write");
 }
 
 void read_encrypted_data(EncryptedData &encrypted_data, const IdashParams &params,
@@ -684,12 +751,12 @@ void read_encrypted_data(EncryptedData &encrypted_data, const IdashParams &param
 
 void write_encrypted_predictions(const EncryptedPredictions &encrypted_preds, const IdashParams &,
                                  const string &filename) {
-  RecordWriter w(filename);
-  REQUIRE_DRAMATICALLY(w.ok(), "Cannot open encrypted predictions file for write");
-  const uint64_t n = encrypted_preds.score.size();
-  w.raw(&n, 8);
-  const int32_t word = stream_word_of(encrypted_preds.block);
-  for (const auto &e : encrypted_preds.score) w.record(e.first, e.second, word);
+  Phases ph("write_encrypted_predictions");
+  std::vector<std::pair<uint32_t, const TLweSample *>> recs(encrypted_preds.score.begin(),
+                                                            encrypted_preds.score.end());
+  write_records(filename, recs, stream_word_of(encrypted_preds.block),
+                "Cannot open encrypted predictions file for write");
+  ph.mark("records written");
 }
 
 void read_encrypted_predictions(EncryptedPredictions &encrypted_preds, const IdashParams &params,
@@ -704,10 +771,23 @@ void read_encrypted_predictions(EncryptedPredictions &encrypted_preds, const Ida
 
 void write_decrypted_predictions(const DecryptedPredictions &predictions, const IdashParams &params,
                                  const string &filename, const bool PRINT_POS_NAME) {
-  FILE *f = std::fopen(filename.c_str(), "wb");
-  REQUIRE_DRAMATICALLY(f, "Cannot open result file for write");
+  Phases ph("write_decrypted_predictions");
+  const int fd = ::open(filename.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+  REQUIRE_DRAMATICALLY(fd >= 0, "Cannot open result file for write");
+  std::atomic<bool> ok{true};
+  auto write_at = [fd, &ok](const char *p, size_t n, off_t off) {
+    while (n) {
+      const ssize_t w = ::pwrite(fd, p, n, off);
+      if (w <= 0) {
+        ok = false;
+        return;
+      }
+      p += w, n -= (size_t)w, off += w;
+    }
+  };
   const char header[] = "Subject ID,target SNP,0,1,2\n";
-  std::fwrite(header, 1, sizeof(header) - 1, f);
+  write_at(header, sizeof(header) - 1, 0);
+  off_t at = sizeof(header) - 1;
   std::vector<const std::array<std::vector<float>, 3> *> rows;
   std::vector<string> labels;
   rows.reserve(params.out_position_names.size());
@@ -715,46 +795,85 @@ void write_decrypted_predictions(const DecryptedPredictions &predictions, const 
     rows.push_back(&predictions.score.at(e.first));
     labels.push_back(PRINT_POS_NAME ? e.second : std::to_string(e.first));
   }
-  // samples are formatted in parallel, one buffer each, and written in order
-  // by a writer thread while the next block of samples is formatted
+  // Samples are formatted in parallel, one buffer each, a block of them at a
+  // time; each buffer's file offset follows from the lengths before it, so
+  // while the next block is formatted, writer threads write this block's
+  // buffers at their offsets. A buffer holds a sample's lines at their longest
+  // (20-digit sample number, three 16-character scores); its pages are touched
+  // only as far as the lines reach.
   const int64_t ns = params.NUM_SAMPLES, block = 64;
-  std::vector<string> text[2] = {std::vector<string>(block), std::vector<string>(block)};
-  std::thread writer;
+  constexpr int kWriters = 4;
+  size_t label_bytes = 0;
+  for (const string &l : labels) label_bytes += l.size();
+  const size_t cap = rows.size() * (20 + 1 + 3 * 17 + 1) + label_bytes;
+  std::vector<std::unique_ptr<char[]>> text[2];
+  for (auto &t : text)
+    for (int64_t i = 0; i < block; ++i) t.emplace_back(new char[cap + 8]);  // + format_g6's scratch
+  std::vector<size_t> length[2] = {std::vector<size_t>(block), std::vector<size_t>(block)};
+  std::vector<off_t> offset[2] = {std::vector<off_t>(block), std::vector<off_t>(block)};
+  std::vector<std::thread> writers;
+  double t_format = 0, t_wait = 0;
+  using clk = std::chrono::steady_clock;
+  // the block's scores gathered position-major ([position][class][sample]),
+  // so that a sample's lines read them at one stride instead of one vector
+  // (and page) per score
+  std::vector<float> tile(rows.size() * 3 * block);
+  ph.mark("setup");
   for (int64_t s0 = 0, buf = 0; s0 < ns; s0 += block, buf ^= 1) {
     const int64_t s1 = std::min(ns, s0 + block);
-    std::vector<string> &tb = text[buf];
+    std::vector<std::unique_ptr<char[]>> &tb = text[buf];
+    std::vector<size_t> &lb = length[buf];
+    const clk::time_point t0 = clk::now();
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < (int64_t)rows.size(); ++t)
+      for (int k = 0; k < 3; ++k)
+        std::memcpy(&tile[((size_t)t * 3 + k) * block], (*rows[t])[k].data() + s0,
+                    (size_t)(s1 - s0) * sizeof(float));
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t s = s0; s < s1; ++s) {
-      string &out = tb[s - s0];
-      out.clear();
-      // "%lld,%s,%g,%g,%g\n"; to_chars' general format with precision 6 is
-      // specified to print as printf's %.6g (= %g)
-      char num[64];
+      // "%lld,%s,%g,%g,%g\n", the scores by format_g6 (printf's %g of a float)
+      char *const begin = tb[s - s0].get();
+      char *p = begin;
+      char num[24];
       const std::to_chars_result sr = std::to_chars(num, num + sizeof(num), (long long)s);
       const size_t slen = (size_t)(sr.ptr - num);
-      for (size_t t = 0; t < rows.size(); ++t) {
-        const auto &pr = *rows[t];
-        out.append(num, slen);
-        out.push_back(',');
-        out.append(labels[t]);
+      const float *sc = tile.data() + (s - s0);
+      for (size_t t = 0; t < rows.size(); ++t, sc += 3 * block) {
+        std::memcpy(p, num, slen);
+        p += slen;
+        *p++ = ',';
+        std::memcpy(p, labels[t].data(), labels[t].size());
+        p += labels[t].size();
         for (int k = 0; k < 3; ++k) {
-          char v[64];
-          v[0] = ',';
-          const std::to_chars_result r = std::to_chars(v + 1, v + sizeof(v), (double)pr[k][s],
-                                                       std::chars_format::general, 6);
-          out.append(v, (size_t)(r.ptr - v));
+          *p++ = ',';
+          p = ihe_host::format_g6(p, sc[k * block]);
         }
-        out.push_back('\n');
+        *p++ = '\n';
       }
+      lb[s - s0] = (size_t)(p - begin);
     }
-    // the writer of the previous block (the other buffer) ends first
-    if (writer.joinable()) writer.join();
-    writer = std::thread([&tb, f, n = s1 - s0] {
-      for (int64_t i = 0; i < n; ++i) std::fwrite(tb[i].data(), 1, tb[i].size(), f);
-    });
+    std::vector<off_t> &ob = offset[buf];
+    for (int64_t i = 0; i < s1 - s0; ++i) {
+      ob[i] = at;
+      at += (off_t)lb[i];
+    }
+    // the previous block's writers (the other buffers) end first
+    const clk::time_point t1 = clk::now();
+    for (std::thread &w : writers) w.join();
+    writers.clear();
+    t_format += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    t_wait += std::chrono::duration<double, std::milli>(clk::now() - t1).count();
+    for (int w = 0; w < kWriters; ++w)
+      writers.emplace_back([&tb, &lb, &ob, &write_at, w, n = s1 - s0] {
+        for (int64_t i = w; i < n; i += kWriters) write_at(tb[i].get(), lb[i], ob[i]);
+      });
   }
-  if (writer.joinable()) writer.join();
-  REQUIRE_DRAMATICALLY(std::fclose(f) == 0, "result file write failed");
+  ph.mark("blocks");
+  for (std::thread &w : writers) w.join();
+  REQUIRE_DRAMATICALLY(::close(fd) == 0 && ok, "result file write failed");
+  ph.mark("last writes, close");
+  ph.note("formatting", t_format);
+  ph.note("waiting for the writes", t_wait);
 }
 
 // ---------------------------------------------------------------------------
@@ -832,64 +951,94 @@ void encrypt_data_ph2(EncryptedData &enc_data, const PlaintextData &plain_data, 
 void cloud_compute_score(EncryptedPredictions &enc_preds, const EncryptedData &enc_data,
                          const Model &model, const IdashParams &params) {
   REQUIRE_DRAMATICALLY(IdashParams::k == 1, "k must be 1");
-  // model -> CSR over output big indices
+  Phases ph("cloud_compute_score");
+  // the model's rows in its own iteration order (which fixes the order of
+  // each row's entries, and so the variances' summation order)
+  using Row = std::unordered_map<FeatBigIndex, int32_t>;
+  std::vector<std::pair<uint32_t, const Row *>> rows;
+  rows.reserve(model.model.size());
   uint32_t num_out = 0;
-  for (const auto &e : model.model) num_out = std::max(num_out, e.first + 1);
+  for (const auto &e : model.model) {
+    rows.push_back({e.first, &e.second});
+    num_out = std::max(num_out, e.first + 1);
+  }
+  // the output block (page-locked, 8 KB per output) is allocated on a second
+  // thread while this one builds the band and runs the step; the context
+  // exists before the thread starts
+  (void)engine_if_present();
+  std::future<std::shared_ptr<SampleBlock>> out_block = std::async(
+      std::launch::async, [num_out] { return std::make_shared<SampleBlock>(num_out); });
+  // model -> CSR over output big indices
   std::vector<uint32_t> row_ptr(num_out + 1, 0);
-  for (const auto &e : model.model) row_ptr[e.first + 1] = (uint32_t)e.second.size();
+  for (const auto &r : rows) row_ptr[r.first + 1] = (uint32_t)r.second->size();
   for (uint32_t o = 0; o < num_out; ++o) row_ptr[o + 1] += row_ptr[o];
   std::vector<uint32_t> in_bidx(row_ptr[num_out]);
   std::vector<int32_t> weight(row_ptr[num_out]);
-  for (const auto &e : model.model) {
-    uint32_t at = row_ptr[e.first];
-    for (const auto &c : e.second) {
-      in_bidx[at] = c.first;
-      weight[at++] = c.second;
-    }
-  }
-  // outputs are created once each (eval/idash.cpp:777-779)
-  for (const auto &e : model.model)
-    if (enc_preds.score.count(e.first)) DIE_DRAMATICALLY("output ciphertext created twice");
   // every ciphertext the model reads must be present (getTLWE,
   // eval/idash.h:162-166), checked before the device is touched
   const size_t cts = num_input_cts(params);
-  {
-    std::vector<char> present(cts, 0);
-    for (const auto &e : enc_data.enc_data)
-      if (e.first < cts) present[e.first] = 1;
-    for (uint32_t b : in_bidx)
-      if (b != params.constant_bigIndex()) {
-        const FeatIndex f = params.feature_indexOf(b);
-        REQUIRE_DRAMATICALLY(f < cts && present[f], "missing input ciphertext");
+  std::vector<char> present(cts, 0);
+  for (const auto &e : enc_data.enc_data)
+    if (e.first < cts) present[e.first] = 1;
+  bool all_present = true;
+#pragma omp parallel for schedule(dynamic, 256) reduction(&& : all_present)
+  for (int64_t i = 0; i < (int64_t)rows.size(); ++i) {
+    uint32_t at = row_ptr[rows[i].first];
+    for (const auto &c : *rows[i].second) {
+      in_bidx[at] = c.first;
+      weight[at++] = c.second;
+      if (c.first != params.constant_bigIndex()) {
+        const FeatIndex f = params.feature_indexOf(c.first);
+        all_present = all_present && f < cts && present[f];
       }
+    }
   }
+  // outputs are created once each (eval/idash.cpp:777-779)
+  for (const auto &r : rows)
+    if (enc_preds.score.count(r.first)) DIE_DRAMATICALLY("output ciphertext created twice");
+  REQUIRE_DRAMATICALLY(all_present, "missing input ciphertext");
+  ph.mark("model CSR and input check");
 
   std::shared_ptr<SampleBlock> in = dense_view(enc_data.enc_data, enc_data.block, cts);
-  auto out = std::make_shared<SampleBlock>(num_out);
-  out->stream_word = stream_word_of(enc_data.block);
+  ph.mark("dense input view");
+  std::shared_ptr<SampleBlock> out;
   if (num_out) {
     ihe_ctx *ctx = engine();
     const ihe_layout lay = layout_of(params);
     const ihe_model_csr csr = {num_out, row_ptr.data(), in_bidx.data(), weight.data()};
     ihe_band *band = nullptr;
     check(ihe_band_create(ctx, &lay, (uint32_t)cts, &csr, IHE_VARIANT_AUTO, &band), "model band");
+    ph.mark("band build and upload");
     {
       DeviceBuffer d_in(cts * kCtBytes), d_in_var(cts * sizeof(double));
       DeviceBuffer d_out(num_out * kCtBytes), d_out_var(num_out * sizeof(double));
+      ph.mark("device allocation");
       if (cts) {
         check(ihe_memcpy_h2d(ctx, d_in.p, in->words(), cts * kCtBytes), "copy");
         check(ihe_memcpy_h2d(ctx, d_in_var.p, in->variances(), cts * sizeof(double)), "copy");
       }
+      ph.mark("inputs to the device");
       check(ihe_cloud_score(ctx, band, d_in.as<int32_t>(), d_in_var.as<double>(), d_out.as<int32_t>(),
                             d_out_var.as<double>()),
             "cloud");
+      check(ihe_sync(ctx), "cloud");
+      ph.mark("cloud step");
+      out = out_block.get();
+      ph.mark("output block (wait)");
       check(ihe_memcpy_d2h(ctx, out->words(), d_out.p, num_out * kCtBytes), "copy");
       check(ihe_memcpy_d2h(ctx, out->variances(), d_out_var.p, num_out * sizeof(double)), "copy");
+      ph.mark("outputs to the host");
     }
     ihe_band_destroy(band);
+    ph.mark("device release");
+  } else {
+    out = out_block.get();
   }
-  for (const auto &e : model.model) enc_preds.score.emplace(e.first, out->sample(e.first));
+  out->stream_word = stream_word_of(enc_data.block);
+  enc_preds.score.reserve(enc_preds.score.size() + model.model.size());
+  for (const auto &r : rows) enc_preds.score.emplace(r.first, out->sample(r.first));
   enc_preds.block = out;
+  ph.mark("prediction map");
 }
 
 // ---------------------------------------------------------------------------
@@ -899,30 +1048,69 @@ void decrypt_predictions(DecryptedPredictions &predictions, const EncryptedPredi
                          const IdashKey &key) {
   const IdashParams &p = *key.idashParams;
   const uint32_t ns = p.NUM_SAMPLES;
-  for (const auto &e : p.out_features_index)
-    for (int snp = 0; snp < 3; ++snp) predictions.score[e.first][snp].resize(ns);
+  Phases ph("decrypt_predictions");
   uint32_t num = 0;
   for (const auto &e : enc_preds.score) num = std::max(num, e.first + 1);
-  for (const auto &e : p.out_features_index)
-    for (FeatBigIndex b : e.second) {
+  // (output big index, its score vector), by index; the map's nodes stay put
+  std::vector<std::pair<FeatBigIndex, std::vector<float> *>> dest;
+  dest.reserve(3 * p.out_features_index.size());
+  for (const auto &e : p.out_features_index) {
+    std::array<std::vector<float>, 3> &v = predictions.score[e.first];
+    for (int snp = 0; snp < 3; ++snp) {
+      const FeatBigIndex b = e.second[snp];
       REQUIRE_DRAMATICALLY(enc_preds.score.count(b), "missing prediction ciphertext");
       num = std::max(num, b + 1);
+      dest.push_back({b, &v[snp]});
     }
+  }
+  std::sort(dest.begin(), dest.end());
+  ph.mark("score vectors");
   if (num == 0) return;
   std::shared_ptr<SampleBlock> in = dense_view(enc_preds.score, enc_preds.block, num);
-  std::vector<float> plain((size_t)num * ns);
-  {
-    ihe_ctx *ctx = engine();
-    DeviceBuffer d_in(num * kCtBytes), d_out(plain.size() * sizeof(float));
-    DeviceKey dk(key.tlweKey);
-    check(ihe_memcpy_h2d(ctx, d_in.p, in->words(), num * kCtBytes), "copy");
-    check(ihe_decrypt(ctx, dk.k, ns, d_in.as<int32_t>(), num, d_out.as<float>()), "decrypt");
-    check(ihe_memcpy_d2h(ctx, plain.data(), d_out.p, plain.size() * sizeof(float)), "copy");
+  ph.mark("dense input view");
+  ihe_ctx *ctx = engine();
+  DeviceBuffer d_in(num * kCtBytes), d_out((size_t)num * ns * sizeof(float));
+  DeviceKey dk(key.tlweKey);
+  check(ihe_memcpy_h2d(ctx, d_in.p, in->words(), num * kCtBytes), "copy");
+  check(ihe_decrypt(ctx, dk.k, ns, d_in.as<int32_t>(), num, d_out.as<float>()), "decrypt");
+  check(ihe_sync(ctx), "decrypt");
+  ph.mark("inputs to the device, decrypt");
+  // The scores come back through two page-locked staging blocks of about 32 MB:
+  // while several threads copy one block's rows into their score vectors (their
+  // storage reserved beforehand, by this thread), the next rows cross the link
+  // into the other block.
+  for (auto &e : dest) e.second->reserve(ns);
+  const size_t row_bytes = (size_t)ns * sizeof(float);
+  const uint32_t rows = (uint32_t)std::max<size_t>(1, std::min<size_t>(num, (32u << 20) / row_bytes));
+  struct Staging {
+    void *p = nullptr;
+    explicit Staging(size_t bytes) { check(ihe_host_alloc(engine(), bytes, &p), "pinned host allocation"); }
+    Staging(const Staging &) = delete;
+    ~Staging() { ihe_host_free(engine(), p); }
+  } stage[2] = {Staging(rows * row_bytes), Staging(rows * row_bytes)};
+  auto fetch = [&](uint32_t r0, int k) {
+    const uint32_t r1 = std::min(num, r0 + rows);
+    check(ihe_memcpy_d2h_async(ctx, stage[k].p, static_cast<const char *>(d_out.p) + r0 * row_bytes,
+                               (r1 - r0) * row_bytes),
+          "copy");
+  };
+  fetch(0, 0);
+  auto d = dest.begin();
+  for (uint32_t r0 = 0, k = 0; r0 < num; r0 += rows, k ^= 1) {
+    const uint32_t r1 = std::min(num, r0 + rows);
+    check(ihe_sync(ctx), "copy");  // rows r0..r1 are in block k
+    if (r1 < num) fetch(r1, k ^ 1);
+    const auto lo = d;
+    while (d != dest.end() && d->first < r1) ++d;
+    const int64_t n = d - lo;
+    const float *block = static_cast<const float *>(stage[k].p);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      const float *src = block + (size_t)(lo[i].first - r0) * ns;
+      lo[i].second->assign(src, src + ns);
+    }
   }
-  for (const auto &e : p.out_features_index)
-    for (int snp = 0; snp < 3; ++snp)
-      std::memcpy(predictions.score[e.first][snp].data(), plain.data() + (size_t)e.second[snp] * ns,
-                  ns * sizeof(float));
+  ph.mark("scores to the host");
 }
 
 // ---------------------------------------------------------------------------

@@ -23,12 +23,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # IHE_HOST_DIR runs the same cases on another build of the executables
 # (make -C idash2019_2_amd/host asan: AddressSanitizer + UBSan)
 HOST = os.path.join(ROOT, os.environ.get("IHE_HOST_DIR", os.path.join("idash2019_2_amd", "host")))
+# the sanitizer build's leak check, less the objects the API leaves to its callers
+LSAN = "suppressions=" + os.path.join(ROOT, "idash2019_2_amd", "host", "lsan.supp")
 ONE, NAN = 1 << 18, (131072, 87381, 43690)
 
 
 def run(exe, *args, cwd):
     r = subprocess.run([os.path.join(HOST, exe), *args], cwd=cwd, capture_output=True, text=True,
-                       timeout=300, env=dict(os.environ, IHE_SEED="9"))
+                       timeout=300, env=dict(os.environ, IHE_SEED="9", LSAN_OPTIONS=LSAN))
     assert r.returncode == 0, f"{exe}: {r.stdout}{r.stderr}"
     return r.stdout
 

@@ -14,13 +14,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # IHE_HOST_DIR runs the same cases on another build of the executables
 # (make -C idash2019_2_amd/host asan: AddressSanitizer + UBSan)
 HOST = os.path.join(ROOT, os.environ.get("IHE_HOST_DIR", os.path.join("idash2019_2_amd", "host")))
+# the sanitizer build's leak check, less the objects the API leaves to its callers
+LSAN = "suppressions=" + os.path.join(ROOT, "idash2019_2_amd", "host", "lsan.supp")
 
 
 def run(exe, *args, cwd, env=None):
     path = os.path.join(HOST, exe)
     if not os.path.exists(path):
         pytest.fail(f"{path} is not built (run __graft_entry__.build())")
-    e = dict(os.environ, IHE_SEED="5", **(env or {}))
+    e = dict(os.environ, IHE_SEED="5", LSAN_OPTIONS=LSAN, **(env or {}))
     return subprocess.run([path, *args], cwd=cwd, env=e, capture_output=True, text=True, timeout=120)
 
 
@@ -226,3 +228,99 @@ def test_sanitizer_build_recipe():
     assert not any(" -fsanitize=" in l.replace("-Xarch_host -fsanitize=", "") for l in hip)
     for exe in ("keygen", "encrypt", "cloud", "decrypt"):
         assert any(f"-o asan/{exe} " in l and "-fsanitize=address,undefined" in l for l in lines), exe
+
+
+DRIVE = os.path.join(ROOT, "tests", "native", "records_drive")
+
+
+def drive(*args):
+    if not os.path.exists(DRIVE):
+        pytest.fail(f"{DRIVE} is not built (run __graft_entry__.build())")
+    return subprocess.run([DRIVE, *args], capture_output=True, text=True, timeout=120)
+
+
+@pytest.mark.parametrize("mode", ["pred", "data"])
+def test_record_files_round_trip(tmp_path, mode):
+    """The host's ciphertext-file reader (mapped a window of records at a time,
+    payloads copied by several threads) and writer (threads writing ranges of
+    records at their file offsets), on 9,000 records in shuffled index order:
+    more than one reader window and many writer ranges. The file written is
+    byte for byte the one a sequential writer makes from the samples read, in
+    the order of the collection's map (eval/idash.cpp:540-556, 596-613)."""
+    n, slots = 9000, 9501                       # some slots stay empty
+    rng = np.random.default_rng(3)
+    idx = rng.permutation(slots)[:n]
+    ct = rng.integers(-(1 << 31), 1 << 31, (slots, 2, 1024), dtype=np.int64).astype(np.int32)
+    var = rng.random(slots)
+    src, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    formats.write_records(str(src), idx, ct, var, -5)
+    count = slots if mode == "pred" else slots // 3   # encrypted data: 3 slots per tag position
+    r = drive(mode, str(src), str(out), str(count))
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.split("\n")
+    assert lines[0] == "word -5"
+    order = np.array([int(x) for x in lines[1:] if x], np.uint32)
+    assert sorted(order.tolist()) == sorted(idx.tolist())
+    want = tmp_path / "want.bin"
+    formats.write_records(str(want), order, ct, var, -5)
+    assert out.read_bytes() == want.read_bytes()
+
+
+def test_record_file_errors(tmp_path):
+    """Truncated files, duplicate indices and indices past the slots end the
+    reader with ERROR (REQUIRE_DRAMATICALLY), checked in file order."""
+    rng = np.random.default_rng(4)
+    ct = rng.integers(-(1 << 31), 1 << 31, (20, 2, 1024), dtype=np.int64).astype(np.int32)
+    var = np.zeros(20)
+    path = tmp_path / "in.bin"
+
+    def run_case(idx, slots, cut=0):
+        if idx:
+            formats.write_records(str(path), idx, ct, var, 0)
+        else:
+            path.write_bytes(bytes(8))                 # a count of zero
+        if cut:
+            path.write_bytes(path.read_bytes()[:-cut])
+        return drive("pred", str(path), str(tmp_path / "out.bin"), str(slots))
+
+    r = run_case([1, 2, 3], 20, cut=100)
+    assert r.returncode != 0 and "ERROR: truncated encrypted predictions file" in r.stdout
+    r = run_case([1, 2, 1], 20)
+    assert r.returncode != 0 and "ERROR: duplicate encrypted predictions index" in r.stdout
+    r = run_case([1, 19, 2], 10)
+    assert r.returncode != 0 and "ERROR: encrypted predictions index out of range" in r.stdout
+    r = run_case([], 10)
+    assert r.returncode == 0 and r.stdout.split("\n")[0] == "word 0"
+    assert (tmp_path / "out.bin").read_bytes() == bytes(8)
+
+
+@pytest.mark.parametrize("names", [0, 1])
+def test_result_file_lines(tmp_path, names):
+    """write_decrypted_predictions (eval/idash.cpp:436-469): the header, then per
+    sample and target position "sample,position or name,s0,s1,s2" with the
+    scores in printf's %g; 150 samples (several formatting blocks, written by
+    several threads at their offsets) over 300 positions."""
+    P, ns = 300, 150
+    rng = np.random.default_rng(5)
+    v = (rng.standard_normal((P, 3, ns)) * 10.0 ** rng.integers(-9, 9, (P, 3, ns))).astype(np.float32)
+    v[0, 0, :4] = [0.0, 1.0, -2.5, 123456789.0]
+    src, out = tmp_path / "scores.bin", tmp_path / "result.csv"
+    v.tofile(src)
+    r = drive("csv", str(src), str(out), str(P), str(ns), str(names))
+    assert r.returncode == 0, r.stdout + r.stderr
+    want = ["Subject ID,target SNP,0,1,2"]
+    for s in range(ns):
+        for i in range(P):
+            label = f"rs{i}" if names else str(1000 + 7 * i)
+            want.append(f"{s},{label}," + ",".join("%g" % float(v[i, k, s]) for k in range(3)))
+    assert out.read_text() == "\n".join(want) + "\n"
+
+
+def test_score_formatter_matches_printf():
+    """The result file's score formatter (host/gfmt.h: exact scaling in 128-bit
+    integers, ties to even) prints what snprintf's %g prints, on every 997th
+    float bit pattern, every multiple of 5 from 10^6 to 2^24 (exact ties at six
+    significant digits, both signs) and each power of ten and its neighbours."""
+    r = drive("gfmt", "997")
+    assert r.returncode == 0 and "mismatches 0" in r.stdout, r.stdout[-2000:]
+    assert int(r.stdout.split("checked ")[1].split()[0]) > 10_000_000
