@@ -350,6 +350,52 @@ def cli_pipeline(workload, log_fn, workdir=None):
     return {"workload": workload, "figures": figures, "file_bytes": sizes}
 
 
+def hbm_probes(eng) -> str:
+    """What HBM sustains on this device for the streaming probe's access
+    patterns (contiguous 16-byte loads and stores, 1 KiB per wave instruction
+    pair) and under each cache policy pair of its loads and stores: the median
+    of 20 launches (HIP events on the launch stream) after 3 warm-ups."""
+    tags, targets = synth.WORKLOADS["full"][:2]
+    cin = 3 * tags * 8192 // 1024 * 1024          # the cloud step's reads
+    cout = 3 * targets * 8192 // 65536 * 65536    # and writes
+    big = ((1 << 31) - (1 << 20)) // 65536 * 65536
+    gib = 1 << 30
+    a, b = eng.empty((big,), np.uint8), eng.empty((big,), np.uint8)
+    a.fill_zero()
+
+    def median_ms(ib, ob, waves, lp, sp):
+        for _ in range(3):
+            eng.stream_probe_ex(a, ib, b, ob, waves, lp, sp)
+        eng.sync()
+        eng.timing(True)
+        times = []
+        for _ in range(20):
+            eng.timing_reset()
+            eng.stream_probe_ex(a, ib, b, ob, waves, lp, sp)
+            times.append(eng.timing_read(_abi.TIMER_STREAM)[0])
+        eng.timing(False)
+        return float(np.median(times))
+
+    lines = ["pattern                                   read MB  write MB   loads/stores   ms       TB/s"]
+    patterns = [("contiguous 16-byte reads", big, 8192 * 1024, 8192),
+                ("contiguous 16-byte writes", 0, big, big // 65536),
+                ("copy (1:1 read:write)", gib, gib, gib // 65536),
+                ("1:5 read:write, the cloud step's volumes", cin, cout, cout // 65536)]
+    for name, ib, ob, waves in patterns:
+        for lp, sp in ((0, 0), (0, 18)):
+            ms = median_ms(ib, ob, waves, lp, sp)
+            lines.append(f"{name:42s}{ib / 1e6:8.1f}{ob / 1e6:10.1f}   {lp:>5}/{sp:<6}  "
+                         f"{ms:8.4f}  {(ib + ob) / ms / 1e9:6.2f}")
+    lines.append("")
+    lines.append("cache policy bits on the cloud step's volumes (sc0 = 1, nt = 2, sc1 = 16)")
+    lines.append("loads/stores   ms")
+    for lp, sp in ((0, 0), (0, 2), (0, 1), (0, 16), (0, 18), (2, 2), (2, 18)):
+        lines.append(f"{lp:>5}/{sp:<6}  {median_ms(cin, cout, cout // 65536, lp, sp):8.4f}")
+    a.free()
+    b.free()
+    return "\n".join(lines) + "\n"
+
+
 def _csv_rows(directory: str, suffix: str):
     """Rows of the rocprofv3 CSV files under `directory` whose names end in `suffix`."""
     import csv
@@ -487,6 +533,9 @@ def main(argv=None):
     ap.add_argument("--cli", action="store_true",
                     help="only run the idash-compatible executables end to end on the workload's "
                          "files and print their timings as JSON")
+    ap.add_argument("--probes", action="store_true",
+                    help="HBM access patterns and cache policies of the streaming probe (a text "
+                         "table on stdout)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (a kernel of known "
                          "traffic for scaling HBM counters) and exit")
@@ -516,6 +565,11 @@ def main(argv=None):
 
     if args.cli:
         print(json.dumps(cli_pipeline(args.workload, log)), flush=True)
+        return 0
+
+    if args.probes:
+        with Engine(0) as eng:
+            sys.stdout.write(hbm_probes(eng))
         return 0
 
     if args.calibrate_copy:

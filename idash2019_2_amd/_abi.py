@@ -97,6 +97,7 @@ SIGNATURES = {
     "ihe_memcpy_d2d": (_int, [_vp, _vp, _vp, _sz]),
     "ihe_checksum": (_int, [_vp, _vp, _sz, ctypes.POINTER(_u64)]),
     "ihe_stream_probe": (_int, [_vp, _vp, _sz, _vp, _sz]),
+    "ihe_stream_probe_ex": (_int, [_vp, _vp, _sz, _vp, _sz, _u32, _u32, _u32]),
     "ihe_timing_enable": (_int, [_vp, _int]),
     "ihe_timing_reset": (_int, [_vp]),
     "ihe_timing_region": (_int, [_vp, _int]),

@@ -309,51 +309,83 @@ __global__ __launch_bounds__(256) void k_checksum(const uint32_t *__restrict__ p
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// One wave per 64 KiB of output: 64 stores of 1 KiB, with the wave's share
-// of the input read in 1 KiB loads spread evenly between them. Stores carry
-// nt | sc1 (18), the fastest documented cache policy for this mix (the
-// cloud kernels' too); loads the default, which beats nt here.
+// Each of num_waves waves writes its share of the output in 1 KiB stores
+// (at least one), with its share of the input read in 1 KiB loads spread
+// evenly between them. Cache policies: LP on the loads, SP on the stores
+// (buffer-instruction bits: sc0 1, nt 2, sc1 16). ihe_stream_probe uses one
+// wave per 64 KiB of output, default loads and nt | sc1 (18) stores, the
+// fastest documented policy for the cloud step's mix (the cloud kernels' too).
+template <int LP, int SP>
 __global__ __launch_bounds__(256) void k_stream_probe(const int *in, uint32_t in_bytes, int *out,
                                                       uint32_t out_bytes, uint32_t num_waves) {
   typedef int int4v __attribute__((ext_vector_type(4)));
   const uint32_t wave = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
   if (wave >= num_waves) return;
-  const uint32_t in_kib = in_bytes / 1024;
+  const uint32_t in_kib = in_bytes / 1024, out_kib = out_bytes / 1024;
   const uint32_t r0 = (uint32_t)((uint64_t)in_kib * wave / num_waves);
   const uint32_t r1 = (uint32_t)((uint64_t)in_kib * (wave + 1) / num_waves);
+  const uint32_t o0 = (uint32_t)((uint64_t)out_kib * wave / num_waves);
+  const uint32_t o1 = (uint32_t)((uint64_t)out_kib * (wave + 1) / num_waves);
   const __amdgpu_buffer_rsrc_t ri =
       __builtin_amdgcn_make_buffer_rsrc((void *)in, (short)0, (int)in_bytes, 0x00020000);
   const __amdgpu_buffer_rsrc_t ro =
       __builtin_amdgcn_make_buffer_rsrc((void *)out, (short)0, (int)out_bytes, 0x00020000);
   int4v acc = {(int)lane, 0, 0, 0};
   uint32_t r = r0;
-  for (uint32_t j = 0; j < 64; ++j) {
-    const uint32_t until = r0 + (uint32_t)((uint64_t)(r1 - r0) * (j + 1) / 64);
-    for (; r < until; ++r) acc += __builtin_amdgcn_raw_buffer_load_b128(ri, r * 1024u + lane * 16u, 0, 0);
-    __builtin_amdgcn_raw_buffer_store_b128(acc, ro, (wave * 64u + j) * 1024u + lane * 16u, 0, 18);
+  for (uint32_t o = o0; o < o1; ++o) {
+    const uint32_t until = r0 + (uint32_t)((uint64_t)(r1 - r0) * (o - o0 + 1) / (o1 - o0));
+    for (; r < until; ++r) acc += __builtin_amdgcn_raw_buffer_load_b128(ri, r * 1024u + lane * 16u, 0, LP);
+    __builtin_amdgcn_raw_buffer_store_b128(acc, ro, o * 1024u + lane * 16u, 0, SP);
   }
+}
+
+template <int LP, int SP>
+void launch_stream(hipStream_t s, const void *in, uint32_t in_bytes, void *out, uint32_t out_bytes,
+                   uint32_t waves) {
+  hipLaunchKernelGGL((k_stream_probe<LP, SP>), dim3((waves + 3) / 4), dim3(256), 0, s,
+                     (const int *)in, in_bytes, (int *)out, out_bytes, waves);
 }
 
 }  // namespace ihe
 
 extern "C" {
 
-int ihe_stream_probe(ihe_ctx *ctx, const void *d_in, size_t in_bytes, void *d_out,
-                     size_t out_bytes) {
+int ihe_stream_probe_ex(ihe_ctx *ctx, const void *d_in, size_t in_bytes, void *d_out,
+                        size_t out_bytes, uint32_t num_waves, uint32_t load_policy,
+                        uint32_t store_policy) {
   if (int rc = check_ctx(ctx)) return rc;
-  if ((in_bytes && !d_in) || !d_out || out_bytes < 65536 || in_bytes % 1024 || out_bytes % 1024 ||
+  if ((in_bytes && !d_in) || !d_out || in_bytes % 1024 || out_bytes % 1024 ||
       in_bytes >= (1ull << 31) || out_bytes >= (1ull << 31) ||
-      ((uintptr_t)d_in | (uintptr_t)d_out) % 16)
-    return fail(IHE_ERR_ARG, "stream probe: sizes must be multiples of 1 KiB below 2^31 "
-                             "(at least 64 KiB written), buffers 16-byte aligned");
+      ((uintptr_t)d_in | (uintptr_t)d_out) % 16 || num_waves == 0 || out_bytes / 1024 < num_waves)
+    return fail(IHE_ERR_ARG, "stream probe: sizes must be multiples of 1 KiB below 2^31, at "
+                             "least 1 KiB written per wave, buffers 16-byte aligned");
   IHE_HIP(hipSetDevice(ctx->device));
-  const uint32_t waves = (uint32_t)(out_bytes / 65536);
+  const hipStream_t s = ctx->stream;
+  const uint32_t ib = (uint32_t)in_bytes, ob = (uint32_t)out_bytes, w = num_waves;
   ScopedTimer t(ctx, IHE_TIMER_STREAM);
-  hipLaunchKernelGGL(k_stream_probe, dim3((waves + 3) / 4), dim3(256), 0, ctx->stream,
-                     (const int *)d_in, (uint32_t)in_bytes, (int *)d_out, (uint32_t)out_bytes,
-                     waves);
+  switch (load_policy * 32 + store_policy) {
+    case 0 * 32 + 0: launch_stream<0, 0>(s, d_in, ib, d_out, ob, w); break;
+    case 0 * 32 + 1: launch_stream<0, 1>(s, d_in, ib, d_out, ob, w); break;
+    case 0 * 32 + 2: launch_stream<0, 2>(s, d_in, ib, d_out, ob, w); break;
+    case 0 * 32 + 16: launch_stream<0, 16>(s, d_in, ib, d_out, ob, w); break;
+    case 0 * 32 + 18: launch_stream<0, 18>(s, d_in, ib, d_out, ob, w); break;
+    case 2 * 32 + 2: launch_stream<2, 2>(s, d_in, ib, d_out, ob, w); break;
+    case 2 * 32 + 18: launch_stream<2, 18>(s, d_in, ib, d_out, ob, w); break;
+    default:
+      return fail(IHE_ERR_ARG, "stream probe: policies (loads, stores) must be one of (0, 0), "
+                               "(0, 1), (0, 2), (0, 16), (0, 18), (2, 2), (2, 18)");
+  }
   IHE_HIP(hipGetLastError());
   return IHE_OK;
+}
+
+int ihe_stream_probe(ihe_ctx *ctx, const void *d_in, size_t in_bytes, void *d_out,
+                     size_t out_bytes) {
+  if (out_bytes < 65536)
+    return fail(IHE_ERR_ARG, "stream probe: sizes must be multiples of 1 KiB below 2^31 "
+                             "(at least 64 KiB written), buffers 16-byte aligned");
+  return ihe_stream_probe_ex(ctx, d_in, in_bytes, d_out, out_bytes, (uint32_t)(out_bytes / 65536),
+                             0, 18);
 }
 
 int ihe_checksum(ihe_ctx *ctx, const void *d_ptr, size_t bytes, uint64_t *out) {

@@ -363,6 +363,17 @@ class Engine:
             raise ValueError("probe sizes exceed the buffers")
         check(lib().ihe_stream_probe(self.ctx, d_in.ptr, in_bytes, d_out.ptr, out_bytes))
 
+    def stream_probe_ex(self, d_in: DeviceArray | None, in_bytes: int, d_out: DeviceArray,
+                        out_bytes: int, num_waves: int, load_policy: int = 0,
+                        store_policy: int = 18) -> None:
+        """ihe_stream_probe_ex: the streaming probe with num_waves waves and the
+        cache-policy bits (sc0 1, nt 2, sc1 16) of its loads and stores."""
+        if d_out.nbytes < out_bytes or (in_bytes and (d_in is None or d_in.nbytes < in_bytes)):
+            raise ValueError("probe sizes exceed the buffers")
+        check(lib().ihe_stream_probe_ex(self.ctx, d_in.ptr if d_in is not None else None, in_bytes,
+                                        d_out.ptr, out_bytes, num_waves, load_policy,
+                                        store_policy))
+
     def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int | None = None) -> None:
         n = min(dst.nbytes, src.nbytes) if nbytes is None else nbytes
         check(lib().ihe_memcpy_d2d(self.ctx, dst.ptr, src.ptr, n))

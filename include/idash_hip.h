@@ -124,6 +124,15 @@ int ihe_checksum(ihe_ctx *ctx, const void *d_ptr, size_t bytes, uint64_t *out);
  * bytes are unspecified. Asynchronous. */
 int ihe_stream_probe(ihe_ctx *ctx, const void *d_in, size_t in_bytes, void *d_out,
                      size_t out_bytes);
+/* The same probe with num_waves waves (each writing at least 1 KiB: out_bytes
+ * >= 1024 * num_waves; in_bytes may be 0 for writes alone, and out_bytes as
+ * small as that for reads alone) and the cache policy bits of its buffer
+ * loads and stores (sc0 = 1, nt = 2, sc1 = 16): (loads, stores) one of (0, 0),
+ * (0, 1), (0, 2), (0, 16), (0, 18), (2, 2), (2, 18). ihe_stream_probe is
+ * num_waves = out_bytes / 65536 with (0, 18). */
+int ihe_stream_probe_ex(ihe_ctx *ctx, const void *d_in, size_t in_bytes, void *d_out,
+                        size_t out_bytes, uint32_t num_waves, uint32_t load_policy,
+                        uint32_t store_policy);
 
 /* ---------------- timing of the library's own kernels ---------------- */
 enum {

@@ -62,6 +62,26 @@ def test_stream_probe_is_timed_and_stays_in_bounds(engine):
             engine.stream_probe(d_in, args[0], d_out, args[1])
 
 
+@pytest.mark.parametrize("policies", [(0, 0), (0, 1), (0, 2), (0, 16), (0, 18), (2, 2), (2, 18)])
+def test_stream_probe_patterns_stay_in_bounds(engine, policies):
+    """ihe_stream_probe_ex: reads alone (1 KiB written per wave), writes alone
+    and a mix, under each cache policy pair; nothing lands past out_bytes."""
+    in_b, guard = 4 << 20, 1 << 16
+    d_in = engine.to_device(np.arange(in_b // 4, dtype=np.uint32))
+    for ib, ob, waves in ((in_b, 1024 * 300, 300), (0, 3 << 20, 700), (in_b, 5 << 20, 80)):
+        d_out = engine.to_device(np.full(ob + guard, 0x5A, np.uint8))
+        engine.stream_probe_ex(d_in, ib, d_out, ob, waves, *policies)
+        out = d_out.numpy()
+        assert (out[ob:] == 0x5A).all()
+        assert (out[:ob] != 0x5A).any()
+        d_out.free()
+    d_out = engine.to_device(np.zeros(4096, np.uint8))
+    for bad in ((in_b, 4096, 5, 0, 18), (in_b, 4096, 4, 1, 18), (in_b, 4096, 0, 0, 18)):
+        with pytest.raises(IheError) as e:
+            engine.stream_probe_ex(d_in, bad[0], d_out, bad[1], bad[2], bad[3], bad[4])
+        assert e.value.code == _abi.IHE_ERR_ARG
+
+
 def test_user_stream_switch(engine):
     """ihe_ctx_set_stream: launches go to a caller's stream (a decrypt and a
     cloud step, equal to the oracle), ihe_ctx_get_stream reports it, and NULL
