@@ -793,6 +793,8 @@ void write_decrypted_predictions(const DecryptedPredictions &predictions, const 
   rows.reserve(params.out_position_names.size());
   for (const auto &e : params.out_position_names) {
     rows.push_back(&predictions.score.at(e.first));
+    for (const std::vector<float> &v : *rows.back())
+      REQUIRE_DRAMATICALLY(v.size() >= params.NUM_SAMPLES, "fewer scores than samples for a position");
     labels.push_back(PRINT_POS_NAME ? e.second : std::to_string(e.first));
   }
   // Samples are formatted in parallel, one buffer each, a block of them at a
