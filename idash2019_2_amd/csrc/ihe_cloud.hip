@@ -1044,9 +1044,29 @@ void interleave_variances(HostBand &H) {
   H.var_w2.swap(w2);
 }
 
+// The grouped model (build_groups' results) of an earlier plan of the same
+// model and layout.
+void copy_groups(const HostBand &G, HostBand &H) {
+  H.bias = G.bias;
+  H.var_ptr = G.var_ptr;
+  H.var_col = G.var_col;
+  H.var_w2 = G.var_w2;
+  H.var_wide_w = G.var_wide_w;
+  H.max_bidx = G.max_bidx;
+  H.has_cols = G.has_cols;
+  H.hg = G.hg;
+  H.gcol = G.gcol;
+  H.gout = G.gout;
+  H.gw = G.gw;
+  H.max_window = G.max_window;
+}
+
 // Validates arguments, picks the kernel family and compiles the band on the host.
+// `grouped`: an earlier plan of the same model and layout whose grouping is
+// reused (the autotuner's candidates differ only in their tiling).
 int plan_band(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr *model,
-              int variant, HostBand &H, const MfmaShape *force_shape = nullptr) {
+              int variant, HostBand &H, const MfmaShape *force_shape = nullptr,
+              const HostBand *grouped = nullptr) {
   if (!model) return fail(IHE_ERR_ARG, "null model");
   if (int rc = check_layout(layout)) return rc;
   if (model->num_outputs > 0 && (!model->row_ptr || (model->row_ptr[model->num_outputs] > 0 &&
@@ -1069,7 +1089,8 @@ int plan_band(const ihe_layout *layout, uint32_t num_in_cts, const ihe_model_csr
       return fail(IHE_ERR_UNSUPPORTED, "MFMA variant needs weights within int16");
   }
   H.wide = wide;
-  if (int rc = build_groups(*layout, num_in_cts, *model, wide, H)) return rc;
+  if (grouped) copy_groups(*grouped, H);
+  else if (int rc = build_groups(*layout, num_in_cts, *model, wide, H)) return rc;
   // MFMA needs weights that split into two signed bytes
   bool mfma_ok = !wide;
   if (mfma_ok) {
@@ -1269,7 +1290,7 @@ int ihe_band_create_tuned(ihe_ctx *ctx, const ihe_layout *layout, uint32_t num_i
   int rc = IHE_OK;
   for (const MfmaShape &cand : std::vector<MfmaShape>(P.candidates)) {
     HostBand H;
-    if ((rc = plan_band(layout, num_in_cts, model, IHE_VARIANT_MFMA, H, &cand))) break;
+    if ((rc = plan_band(layout, num_in_cts, model, IHE_VARIANT_MFMA, H, &cand, &P))) break;
     ihe_band *b = nullptr;
     if ((rc = upload_band(ctx, layout, num_in_cts, model, H, &b))) break;
     int best_ntw = 0;
