@@ -34,6 +34,7 @@
 #include <cstring>
 #include <iterator>
 #include <string>
+#include <thread>
 #include <unordered_map>
 
 #include "ihe_internal.h"
@@ -785,6 +786,19 @@ void layout_staged(const ihe_layout &L, uint32_t tile_outputs, HostBand &H) {
 }
 
 
+// Runs fn(begin, end) over [0, n) in contiguous ranges on up to 16 threads
+// (one when n is small).
+template <typename F>
+void parallel_ranges(size_t n, F fn) {
+  const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t T = std::min(hw, (n + 63) / 64);
+  if (T <= 1) return fn((size_t)0, n);
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < T; ++t) th.emplace_back(fn, n * t / T, n * (t + 1) / T);
+  fn((size_t)0, n / T);
+  for (std::thread &x : th) x.join();
+}
+
 // MFMA layout: tiles of 32 consecutive output slots over the union of their
 // columns. Returns false when a tile's union exceeds kMfmaMaxKb blocks.
 // MFMA layout: tiles of up to 32*RB consecutive rows whose column union fits
@@ -840,23 +854,44 @@ bool layout_mfma(uint32_t num_out, const MfmaShape &shape, HostBand &H) {
   sb.close(ntiles);
 
   H.shape = shape;
-  for (const Segment &seg : H.segments) {
-    for (uint32_t t = seg.tile_begin; t < seg.tile_end; ++t) {
+  // each tile's slices of the device arrays, then the tiles filled in parallel
+  const uint32_t rb_count = (uint32_t)shape.rb;
+  std::vector<uint32_t> tile_nkb(ntiles), tile_seg(ntiles);
+  std::vector<size_t> kcol_at(ntiles + 1), a_at(ntiles + 1);
+  const size_t kcol0 = H.kcols.size(), a0 = H.adata.size(), row0 = H.rowc.size(), tile0 = H.mtiles.size();
+  size_t ncols = 0;
+  for (uint32_t i = 0; i < (uint32_t)H.segments.size(); ++i)
+    for (uint32_t t = H.segments[i].tile_begin; t < H.segments[i].tile_end; ++t) tile_seg[t] = i;
+  kcol_at[0] = kcol0, a_at[0] = a0;
+  for (uint32_t t = 0; t < ntiles; ++t) {
+    tile_nkb[t] = std::max<uint32_t>(1, (uint32_t)(U[t].size() + 31) / 32);
+    kcol_at[t + 1] = kcol_at[t] + 32 * tile_nkb[t];
+    a_at[t + 1] = a_at[t] + (size_t)rb_count * tile_nkb[t] * 2 * 64;
+    ncols += U[t].size();
+  }
+  H.kcols.resize(kcol_at[ntiles]);
+  H.adata.resize(a_at[ntiles]);
+  H.rowc.resize(row0 + (size_t)rows * ntiles);
+  H.mtiles.resize(tile0 + ntiles);
+  H.cols.resize(H.cols.size() + ncols);  // counts tile columns for the band info
+  parallel_ranges(ntiles, [&](size_t t_begin, size_t t_end) {
+    std::vector<int32_t> W;
+    for (uint32_t t = (uint32_t)t_begin; t < (uint32_t)t_end; ++t) {
+      const Segment &seg = H.segments[tile_seg[t]];
       const std::vector<uint32_t> &u = U[t];
-      const uint32_t o0 = t_o0[t], nrows = t_rows[t];
-      const uint32_t nkb = std::max<uint32_t>(1, (uint32_t)(u.size() + 31) / 32);
+      const uint32_t o0 = t_o0[t], nrows = t_rows[t], nkb = tile_nkb[t];
       MTileDev D{};
-      D.kcol_off = (uint32_t)H.kcols.size();
+      D.kcol_off = (uint32_t)kcol_at[t];
       D.nkb = nkb;
-      D.a_off = (uint32_t)(H.adata.size() / 64);
-      D.row_off = (uint32_t)H.rowc.size();
+      D.a_off = (uint32_t)(a_at[t] / 64);
+      D.row_off = (uint32_t)(row0 + (size_t)rows * t);
       D.slot0 = o0 - seg.out_base;
       D.nrows = nrows;
       // padding columns repeat a real one (or the segment base) with weight 0
       const uint32_t fill = u.empty() ? seg.in_base : u.back();
-      for (uint32_t k = 0; k < 32 * nkb; ++k) H.kcols.push_back(k < u.size() ? u[k] : fill);
+      for (uint32_t k = 0; k < 32 * nkb; ++k) H.kcols[kcol_at[t] + k] = k < u.size() ? u[k] : fill;
       // dense weights of the tile: W[m][k]
-      std::vector<int32_t> W((size_t)rows * 32 * nkb, 0);
+      W.assign((size_t)rows * 32 * nkb, 0);
       for (uint32_t m = 0; m < nrows; ++m) {
         const HostGroup &hg = H.hg[og[o0 + m]];
         const uint32_t *gc = H.gcol.data() + hg.col_begin;
@@ -867,7 +902,8 @@ bool layout_mfma(uint32_t num_out, const MfmaShape &shape, HostBand &H) {
         }
       }
       // A operand blocks in (row block, K-block, weight byte) order
-      for (uint32_t rb = 0; rb < (uint32_t)shape.rb; ++rb)
+      size_t a = a_at[t];
+      for (uint32_t rb = 0; rb < rb_count; ++rb)
         for (uint32_t kb = 0; kb < nkb; ++kb)
           for (uint32_t hp = 0; hp < 2; ++hp)
             for (uint32_t lane = 0; lane < 64; ++lane) {
@@ -878,15 +914,12 @@ bool layout_mfma(uint32_t num_out, const MfmaShape &shape, HostBand &H) {
                 const int32_t w0 = (int8_t)(w & 0xFF);
                 bytes[j] = (int8_t)(hp == 0 ? w0 : (w - w0) / 256);
               }
-              int4v a;
-              std::memcpy(&a, bytes, 16);
-              H.adata.push_back(a);
+              std::memcpy(&H.adata[a++], bytes, 16);
             }
-      for (uint32_t m = 0; m < rows; ++m) H.rowc.push_back(m < nrows ? H.bias[o0 + m] : 0u);
-      H.mtiles.push_back(D);
-      H.cols.resize(H.cols.size() + u.size());  // counts tile columns for the band info
+      for (uint32_t m = 0; m < rows; ++m) H.rowc[D.row_off + m] = m < nrows ? H.bias[o0 + m] : 0u;
+      H.mtiles[tile0 + t] = D;
     }
-  }
+  });
   return true;
 }
 
