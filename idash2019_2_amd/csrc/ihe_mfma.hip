@@ -21,6 +21,8 @@
 // of the result in lane l is row 8*(v/4) + 4*(l/32) + v%4, word l%32.
 #include "ihe_mfma.h"
 
+#include <cstdlib>
+
 namespace ihe {
 namespace {
 
@@ -585,6 +587,113 @@ void k_cloud_mfmaw(MfmaArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// Unrotated 128-row tiles at three workgroups per CU. The product is
+// k_cloud_mfmaw's; the input words reach LDS by LDS-direct buffer loads (no
+// load registers) and the column words stay in LDS, so that a wave fits the
+// 168 registers of three waves per SIMD. Per column tile: the workgroup
+// recodes the staged words into the B planes (one buffer), then issues the
+// next column tile's loads straight into the staging buffer and multiplies
+// while they arrive; the other workgroups of the CU fill each one's barriers.
+// s_raw: [NKB * 32 columns][32 words]; load instruction q of the tile writes
+// columns 2q (lanes 0..31) and 2q + 1 (lanes 32..63).
+template <int NTW, int NKB>
+__device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, int4v *s_b,
+                                               uint32_t *s_raw, uint32_t *s_cw, uint32_t *s_bias,
+                                               uint32_t t) {
+  constexpr uint32_t wg_per_tile = kCtWords / (NTW * 32);
+  constexpr uint32_t rows = kMfmaRows * kWaves;
+  uint32_t tile;
+  const MfmaEntry E = entry_of(A, blk / wg_per_tile, tile);
+  const uint32_t nt0 = (blk % wg_per_tile) * NTW;
+  auto col = [&](uint32_t nt) { return nt0 + nt; };
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(t / 64), lane = t % 64;
+  const uint32_t h = lane / 32, r = lane % 32;
+  const MTileDev T = tile_of(E, tile);
+  const __amdgpu_buffer_rsrc_t in_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)E.in, (short)0, (int)E.in_bytes, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t out_r =
+      __builtin_amdgcn_make_buffer_rsrc((void *)E.out, (short)0, (int)E.out_bytes, kRsrcWord3);
+  if (t < rows) s_bias[t] = E.rowc[T.row_off + t];
+  // column words; K-blocks past the tile's own repeat its last one (zero weights)
+  for (uint32_t i = t; i < NKB * 32; i += 64 * kWaves) {
+    const uint32_t kb = i / 32, kbs = kb < T.nkb ? kb : T.nkb - 1;
+    s_cw[i] = col_word<false>(E.kcols[T.kcol_off + 32 * kbs + i % 32], E);
+  }
+  int4v a[NKB][2];
+#pragma unroll
+  for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+    for (int hp = 0; hp < 2; ++hp)
+      a[kb][hp] = (uint32_t)kb < T.nkb
+                      ? E.adata[(size_t)(T.a_off + (wave * T.nkb + kb) * 2 + hp) * 64 + lane]
+                      : int4v{0, 0, 0, 0};
+  const uint32_t row_base = T.slot0 * kCtWords * 4u;
+  const bool full = T.nrows == rows;
+  typedef __attribute__((address_space(3))) void *lds_ptr;
+  auto issue_raw = [&](uint32_t nt) {
+#pragma unroll
+    for (int i = 0; i < NKB * 4; ++i) {
+      const uint32_t q = (uint32_t)i * kWaves + wave;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(in_r, (lds_ptr)(s_raw + q * 64), 4,
+                                               s_cw[2 * q + h] + col(nt) * 128u + r * 4u, 0, 0, 0);
+    }
+  };
+  // task k: K-block k / 64 and B lane k % 64 (half, word) of every plane
+  auto recode = [&]() {
+    for (uint32_t k = t; k < NKB * 64; k += 64 * kWaves) {
+      const uint32_t kb = k / 64, bl = k % 64;
+      const uint32_t *src = s_raw + (kb * 32 + 16 * (bl / 32)) * 32 + bl % 32;
+      uint32_t x[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) x[j] = src[32 * j];
+      int4v pl[4];
+      byte_planes(x, pl);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) s_b[(kb * 4 + b) * 64 + bl] = pl[b];
+    }
+  };
+  __syncthreads();
+  issue_raw(0);
+  for (uint32_t nt = 0; nt < (uint32_t)NTW; ++nt) {
+    // this wave's staging loads have landed (the previous tile's 16 stores,
+    // issued after them, may still be in flight), then every wave's have
+    if (nt == 0) __builtin_amdgcn_s_waitcnt(0x0F70);        // vmcnt(0)
+    else __builtin_amdgcn_s_waitcnt(0x4F70);                // vmcnt(16)
+    __syncthreads();
+    recode();
+    __syncthreads();
+    if (nt + 1 < (uint32_t)NTW) issue_raw(nt + 1);
+    Acc c = {};
+    int4v pl[2][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pl[0][b] = s_b[b * 64 + lane];
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) {
+      if (kb + 1 < NKB) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) pl[(kb + 1) & 1][b] = s_b[((kb + 1) * 4 + b) * 64 + lane];
+      }
+      mfma_kblock(a[kb][0], a[kb][1], pl[kb & 1], c);
+    }
+    uint32_t rbase = row_base;
+    asm volatile("" : "+s"(rbase));
+    finish_rows<false>(c, E, out_r, s_bias, col(nt) * 32, r, h, 32 * wave, T.nrows, full, rbase);
+  }
+}
+
+template <int NTW, int NKB>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(3)))
+void k_cloud_mfmaw_lds(MfmaArgs A) {
+  __shared__ int4v s_b[NKB * 4 * 64];
+  __shared__ uint32_t s_raw[NKB * 32 * 32];
+  __shared__ uint32_t s_cw[NKB * 32];
+  __shared__ uint32_t s_bias[kMfmaRows * kWaves];
+  if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
+  mfmaw_lds_body<NTW, NKB>(A, xcd_block(blockIdx.x, A.cloud_blocks), s_b, s_raw, s_cw, s_bias,
+                           threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
 // A whole batch in one launch. The grid's first workgroups each run one block
 // of the one-K-block families (32- or 64-row tiles): those blocks are short
 // and store-bound, and run as in their own kernel when the dispatcher hands
@@ -822,6 +931,15 @@ void launch_kk(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
 template <bool ROT, int NTW>
 void launch_kw(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
   const dim3 block(64 * kWaves);
+  const char *e = std::getenv("IHE_MFMAW_LDS");
+  const bool lds = !(e && e[0] == '0');
+  if (!ROT && lds && nkb <= 6) {
+    switch ((nkb + 1) / 2) {
+      case 1: hipLaunchKernelGGL((k_cloud_mfmaw_lds<NTW, 2>), grid, block, 0, s, A); return;
+      case 2: hipLaunchKernelGGL((k_cloud_mfmaw_lds<NTW, 4>), grid, block, 0, s, A); return;
+      default: hipLaunchKernelGGL((k_cloud_mfmaw_lds<NTW, 6>), grid, block, 0, s, A); return;
+    }
+  }
   switch ((nkb + 1) / 2) {
     case 1: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 2>), grid, block, 0, s, A); break;
     case 2: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 4>), grid, block, 0, s, A); break;

@@ -12,7 +12,9 @@ pytestmark = pytest.mark.gpu
 # The cloud kernels. MFMA shapes (IHE_CLOUD_SHAPE = row blocks, K-blocks):
 # 64-row single-K-block tiles (the default at 5 neighbours), 32-row tiles,
 # and 2 or 3 K-blocks (forcing padded K-blocks on narrow windows); NTW = 32-word
-# column tiles per wave. Then the LDS-staged dot2 VALU and direct dot2 VALU.
+# column tiles per wave. 128-row tiles ("w") run the LDS-staged kernel when
+# unrotated, up to 6 K-blocks, and the register-staged one otherwise
+# (IHE_MFMAW_LDS=0 forces it). Then the LDS-staged dot2 VALU and direct dot2 VALU.
 KERNELS = {
     "mfma": {},
     "mfma-rb1": {"IHE_CLOUD_SHAPE": "1,1"},
@@ -22,6 +24,8 @@ KERNELS = {
     "mfma-rb2": {"IHE_CLOUD_SHAPE": "2,1"},
     "mfma-w1": {"IHE_CLOUD_SHAPE": "4,1", "IHE_CLOUD_NTW": "8"},
     "mfma-w2": {"IHE_CLOUD_SHAPE": "4,2"},
+    "mfma-w2-regs": {"IHE_CLOUD_SHAPE": "4,2", "IHE_MFMAW_LDS": "0"},
+    "mfma-w6": {"IHE_CLOUD_SHAPE": "4,6", "IHE_CLOUD_NTW": "16"},
     "mfma-w7": {"IHE_CLOUD_SHAPE": "4,7", "IHE_CLOUD_NTW": "32"},
     "staged": {"IHE_CLOUD_KERNEL": "staged"},
     "direct": {"IHE_CLOUD_KERNEL": "direct"},
@@ -30,7 +34,7 @@ KERNELS = {
 
 @pytest.fixture(params=list(KERNELS))
 def kernel(request, monkeypatch):
-    for k in ("IHE_CLOUD_KERNEL", "IHE_CLOUD_SHAPE", "IHE_CLOUD_NTW"):
+    for k in ("IHE_CLOUD_KERNEL", "IHE_CLOUD_SHAPE", "IHE_CLOUD_NTW", "IHE_MFMAW_LDS"):
         monkeypatch.delenv(k, raising=False)
     for k, v in KERNELS[request.param].items():
         monkeypatch.setenv(k, v)
