@@ -587,16 +587,18 @@ void k_cloud_mfmaw(MfmaArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// Unrotated 128-row tiles at three workgroups per CU. The product is
-// k_cloud_mfmaw's; the input words reach LDS by LDS-direct buffer loads (no
-// load registers) and the column words stay in LDS, so that a wave fits the
-// 168 registers of three waves per SIMD. Per column tile: the workgroup
-// recodes the staged words into the B planes (one buffer), then issues the
-// next column tile's loads straight into the staging buffer and multiplies
-// while they arrive; the other workgroups of the CU fill each one's barriers.
+// 128-row tiles at three workgroups per CU. The product is k_cloud_mfmaw's;
+// the input words reach LDS by LDS-direct buffer loads (no load registers) and
+// the column words stay in LDS, so that a wave fits the 168 registers of three
+// waves per SIMD. A rotated column loads its words from their rotated places,
+// and the recoding negates those that wrapped past N. Per column tile: the
+// workgroup recodes the staged words into the B planes (one buffer), then
+// issues the next column tile's loads straight into the staging buffer and
+// multiplies while they arrive; the other workgroups of the CU fill each one's
+// barriers.
 // s_raw: [NKB * 32 columns][32 words]; load instruction q of the tile writes
 // columns 2q (lanes 0..31) and 2q + 1 (lanes 32..63).
-template <int NTW, int NKB>
+template <bool ROT, int NTW, int NKB>
 __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, int4v *s_b,
                                                uint32_t *s_raw, uint32_t *s_cw, uint32_t *s_bias,
                                                uint32_t t) {
@@ -617,7 +619,7 @@ __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, 
   // column words; K-blocks past the tile's own repeat its last one (zero weights)
   for (uint32_t i = t; i < NKB * 32; i += 64 * kWaves) {
     const uint32_t kb = i / 32, kbs = kb < T.nkb ? kb : T.nkb - 1;
-    s_cw[i] = col_word<false>(E.kcols[T.kcol_off + 32 * kbs + i % 32], E);
+    s_cw[i] = col_word<ROT>(E.kcols[T.kcol_off + 32 * kbs + i % 32], E);
   }
   int4v a[NKB][2];
 #pragma unroll
@@ -634,18 +636,36 @@ __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, 
 #pragma unroll
     for (int i = 0; i < NKB * 4; ++i) {
       const uint32_t q = (uint32_t)i * kWaves + wave;
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(in_r, (lds_ptr)(s_raw + q * 64), 4,
-                                               s_cw[2 * q + h] + col(nt) * 128u + r * 4u, 0, 0, 0);
+      const uint32_t cw = s_cw[2 * q + h];
+      uint32_t off;
+      if constexpr (!ROT) {
+        off = cw + col(nt) * 128u + r * 4u;
+      } else {  // load_col's address of word col(nt) * 32 + r
+        const uint32_t word = col(nt) * 32 + r, tt = (word & (kN - 1)) + ((cw >> 2) & (kN - 1));
+        off = (cw & ~0x1FFFu) + ((tt & (kN - 1)) << 2) + (word & kN) * 4u;
+      }
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(in_r, (lds_ptr)(s_raw + q * 64), 4, off, 0, 0, 0);
     }
   };
   // task k: K-block k / 64 and B lane k % 64 (half, word) of every plane
-  auto recode = [&]() {
+  auto recode = [&](uint32_t nt) {
     for (uint32_t k = t; k < NKB * 64; k += 64 * kWaves) {
       const uint32_t kb = k / 64, bl = k % 64;
-      const uint32_t *src = s_raw + (kb * 32 + 16 * (bl / 32)) * 32 + bl % 32;
+      const uint32_t c0 = kb * 32 + 16 * (bl / 32);
+      const uint32_t *src = s_raw + c0 * 32 + bl % 32;
       uint32_t x[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) x[j] = src[32 * j];
+      if constexpr (ROT) {  // words that wrapped past N enter negated
+        const uint32_t wn = (col(nt) * 32 + bl % 32) & (kN - 1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const u32x4v cv = *(const u32x4v *)(s_cw + c0 + 4 * q);
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (wn + ((cv[i] >> 2) & (kN - 1)) >= kN) x[4 * q + i] = 0u - x[4 * q + i];
+        }
+      }
       int4v pl[4];
       byte_planes(x, pl);
 #pragma unroll
@@ -660,7 +680,7 @@ __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, 
     if (nt == 0) __builtin_amdgcn_s_waitcnt(0x0F70);        // vmcnt(0)
     else __builtin_amdgcn_s_waitcnt(0x4F70);                // vmcnt(16)
     __syncthreads();
-    recode();
+    recode(nt);
     __syncthreads();
     if (nt + 1 < (uint32_t)NTW) issue_raw(nt + 1);
     Acc c = {};
@@ -681,16 +701,16 @@ __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, 
   }
 }
 
-template <int NTW, int NKB>
+template <bool ROT, int NTW, int NKB>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(3)))
 void k_cloud_mfmaw_lds(MfmaArgs A) {
   __shared__ int4v s_b[NKB * 4 * 64];
-  __shared__ uint32_t s_raw[NKB * 32 * 32];
-  __shared__ uint32_t s_cw[NKB * 32];
+  __shared__ __attribute__((aligned(16))) uint32_t s_raw[NKB * 32 * 32];
+  __shared__ __attribute__((aligned(16))) uint32_t s_cw[NKB * 32];
   __shared__ uint32_t s_bias[kMfmaRows * kWaves];
   if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
-  mfmaw_lds_body<NTW, NKB>(A, xcd_block(blockIdx.x, A.cloud_blocks), s_b, s_raw, s_cw, s_bias,
-                           threadIdx.x);
+  mfmaw_lds_body<ROT, NTW, NKB>(A, xcd_block(blockIdx.x, A.cloud_blocks), s_b, s_raw, s_cw,
+                                s_bias, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -933,11 +953,11 @@ void launch_kw(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
   const dim3 block(64 * kWaves);
   const char *e = std::getenv("IHE_MFMAW_LDS");
   const bool lds = !(e && e[0] == '0');
-  if (!ROT && lds && nkb <= 6) {
+  if (lds && nkb <= 6) {
     switch ((nkb + 1) / 2) {
-      case 1: hipLaunchKernelGGL((k_cloud_mfmaw_lds<NTW, 2>), grid, block, 0, s, A); return;
-      case 2: hipLaunchKernelGGL((k_cloud_mfmaw_lds<NTW, 4>), grid, block, 0, s, A); return;
-      default: hipLaunchKernelGGL((k_cloud_mfmaw_lds<NTW, 6>), grid, block, 0, s, A); return;
+      case 1: hipLaunchKernelGGL((k_cloud_mfmaw_lds<ROT, NTW, 2>), grid, block, 0, s, A); return;
+      case 2: hipLaunchKernelGGL((k_cloud_mfmaw_lds<ROT, NTW, 4>), grid, block, 0, s, A); return;
+      default: hipLaunchKernelGGL((k_cloud_mfmaw_lds<ROT, NTW, 6>), grid, block, 0, s, A); return;
     }
   }
   switch ((nkb + 1) / 2) {

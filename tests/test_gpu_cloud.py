@@ -12,9 +12,9 @@ pytestmark = pytest.mark.gpu
 # The cloud kernels. MFMA shapes (IHE_CLOUD_SHAPE = row blocks, K-blocks):
 # 64-row single-K-block tiles (the default at 5 neighbours), 32-row tiles,
 # and 2 or 3 K-blocks (forcing padded K-blocks on narrow windows); NTW = 32-word
-# column tiles per wave. 128-row tiles ("w") run the LDS-staged kernel when
-# unrotated, up to 6 K-blocks, and the register-staged one otherwise
-# (IHE_MFMAW_LDS=0 forces it). Then the LDS-staged dot2 VALU and direct dot2 VALU.
+# column tiles per wave. 128-row tiles ("w") run the LDS-staged kernel up to 6
+# K-blocks and the register-staged one above (IHE_MFMAW_LDS=0 forces it). Then
+# the LDS-staged dot2 VALU and direct dot2 VALU.
 KERNELS = {
     "mfma": {},
     "mfma-rb1": {"IHE_CLOUD_SHAPE": "1,1"},
