@@ -596,9 +596,11 @@ void k_cloud_mfmaw(MfmaArgs A) {
 // issues the next column tile's loads straight into the staging buffer and
 // multiplies while they arrive; the other workgroups of the CU fill each one's
 // barriers.
-// s_raw: [NKB * 32 columns][32 words]; load instruction q of the tile writes
-// columns 2q (lanes 0..31) and 2q + 1 (lanes 32..63).
-template <bool ROT, int NTW, int NKB>
+// s_raw: [NKB * 32 columns][32 words]. With dword loads, load instruction q of
+// the tile writes columns 2q (lanes 0..31) and 2q + 1 (lanes 32..63); with
+// 16-byte loads (W16: unrotated only, where a column's 32 words are 128
+// consecutive bytes) it writes columns 8q..8q + 7, four words per lane.
+template <bool ROT, bool W16, int NTW, int NKB>
 __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, int4v *s_b,
                                                uint32_t *s_raw, uint32_t *s_cw, uint32_t *s_bias,
                                                uint32_t t) {
@@ -632,7 +634,19 @@ __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, 
   const uint32_t row_base = T.slot0 * kCtWords * 4u;
   const bool full = T.nrows == rows;
   typedef __attribute__((address_space(3))) void *lds_ptr;
+  static_assert(!(ROT && W16), "a rotated column's words are not 16-byte runs");
+  static_assert((NKB * 4) % kWaves == 0, "16-byte staging loads split evenly over the waves");
   auto issue_raw = [&](uint32_t nt) {
+    if constexpr (W16) {
+#pragma unroll
+      for (int i = 0; i < NKB * 4 / kWaves; ++i) {
+        const uint32_t q = (uint32_t)i * kWaves + wave;
+        const uint32_t off = s_cw[8 * q + lane / 8] + col(nt) * 128u + (lane % 8) * 16u;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(in_r, (lds_ptr)(s_raw + q * 256), 16, off, 0, 0,
+                                                 0);
+      }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < NKB * 4; ++i) {
       const uint32_t q = (uint32_t)i * kWaves + wave;
@@ -701,7 +715,7 @@ __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, 
   }
 }
 
-template <bool ROT, int NTW, int NKB>
+template <bool ROT, bool W16, int NTW, int NKB>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(3)))
 void k_cloud_mfmaw_lds(MfmaArgs A) {
   __shared__ int4v s_b[NKB * 4 * 64];
@@ -709,8 +723,8 @@ void k_cloud_mfmaw_lds(MfmaArgs A) {
   __shared__ __attribute__((aligned(16))) uint32_t s_cw[NKB * 32];
   __shared__ uint32_t s_bias[kMfmaRows * kWaves];
   if (blockIdx.x >= A.cloud_blocks) return var_block(A.var, blockIdx.x - A.cloud_blocks);
-  mfmaw_lds_body<ROT, NTW, NKB>(A, xcd_block(blockIdx.x, A.cloud_blocks), s_b, s_raw, s_cw,
-                                s_bias, threadIdx.x);
+  mfmaw_lds_body<ROT, W16, NTW, NKB>(A, xcd_block(blockIdx.x, A.cloud_blocks), s_b, s_raw, s_cw,
+                                     s_bias, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -946,19 +960,29 @@ void launch_kk(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
   }
 }
 
+template <bool ROT, bool W16, int NTW>
+void launch_kw_lds(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
+  const dim3 block(64 * kWaves);
+  switch ((nkb + 1) / 2) {
+    case 1: hipLaunchKernelGGL((k_cloud_mfmaw_lds<ROT, W16, NTW, 2>), grid, block, 0, s, A); break;
+    case 2: hipLaunchKernelGGL((k_cloud_mfmaw_lds<ROT, W16, NTW, 4>), grid, block, 0, s, A); break;
+    default: hipLaunchKernelGGL((k_cloud_mfmaw_lds<ROT, W16, NTW, 6>), grid, block, 0, s, A); break;
+  }
+}
+
 // tiles of nkb K-blocks run on the kernel with the next even NKB (padding
-// K-blocks carry zero weights)
+// K-blocks carry zero weights). IHE_MFMAW_LDS: 0 = register-staged kernel,
+// 1 = LDS-staged with dword staging loads, else LDS-staged with 16-byte
+// staging loads where the layout is unrotated.
 template <bool ROT, int NTW>
 void launch_kw(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
   const dim3 block(64 * kWaves);
   const char *e = std::getenv("IHE_MFMAW_LDS");
-  const bool lds = !(e && e[0] == '0');
-  if (lds && nkb <= 6) {
-    switch ((nkb + 1) / 2) {
-      case 1: hipLaunchKernelGGL((k_cloud_mfmaw_lds<ROT, NTW, 2>), grid, block, 0, s, A); return;
-      case 2: hipLaunchKernelGGL((k_cloud_mfmaw_lds<ROT, NTW, 4>), grid, block, 0, s, A); return;
-      default: hipLaunchKernelGGL((k_cloud_mfmaw_lds<ROT, NTW, 6>), grid, block, 0, s, A); return;
-    }
+  const char mode = e && (e[0] == '0' || e[0] == '1') ? e[0] : '2';
+  if (mode != '0' && nkb <= 6) {
+    if (!ROT && mode == '2') launch_kw_lds<false, true, NTW>(s, nkb, grid, A);
+    else launch_kw_lds<ROT, false, NTW>(s, nkb, grid, A);
+    return;
   }
   switch ((nkb + 1) / 2) {
     case 1: hipLaunchKernelGGL((k_cloud_mfmaw<ROT, NTW, 2>), grid, block, 0, s, A); break;

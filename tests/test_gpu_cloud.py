@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 # 64-row single-K-block tiles (the default at 5 neighbours), 32-row tiles,
 # and 2 or 3 K-blocks (forcing padded K-blocks on narrow windows); NTW = 32-word
 # column tiles per wave. 128-row tiles ("w") run the LDS-staged kernel up to 6
-# K-blocks and the register-staged one above (IHE_MFMAW_LDS=0 forces it). Then
+# K-blocks (16-byte staging loads where unrotated; IHE_MFMAW_LDS=1 forces dword
+# loads) and the register-staged one above (IHE_MFMAW_LDS=0 forces it). Then
 # the LDS-staged dot2 VALU and direct dot2 VALU.
 KERNELS = {
     "mfma": {},
@@ -25,6 +26,7 @@ KERNELS = {
     "mfma-w1": {"IHE_CLOUD_SHAPE": "4,1", "IHE_CLOUD_NTW": "8"},
     "mfma-w2": {"IHE_CLOUD_SHAPE": "4,2"},
     "mfma-w2-regs": {"IHE_CLOUD_SHAPE": "4,2", "IHE_MFMAW_LDS": "0"},
+    "mfma-w4-dword": {"IHE_CLOUD_SHAPE": "4,4", "IHE_MFMAW_LDS": "1"},
     "mfma-w6": {"IHE_CLOUD_SHAPE": "4,6", "IHE_CLOUD_NTW": "16"},
     "mfma-w7": {"IHE_CLOUD_SHAPE": "4,7", "IHE_CLOUD_NTW": "32"},
     "staged": {"IHE_CLOUD_KERNEL": "staged"},

@@ -155,17 +155,20 @@ def test_small_workload_through_the_executables(tmp_path):
 # ---------------------------------------------------------------- wide
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("kernel", ["auto", "registers", "4,8", "staged"])
+@pytest.mark.parametrize("kernel", ["auto", "dword", "registers", "4,8", "staged"])
 def test_wide_workload_full_layout(engine, kernel, monkeypatch):
     """n = 50 on the full layout: 150 one-hot columns per output, 242,646
     outputs from 48,552 device-encrypted inputs; the MFMA kernel (the default
-    for wide windows) on its own choice of tile (the LDS-staged 128-row kernel,
-    and the register-staged one), on 128-row x 8-K-block tiles (two K-blocks
-    of padding on most), and the LDS-staged VALU kernel."""
+    for wide windows) on its own choice of tile (the LDS-staged 128-row kernel
+    with 16-byte and with dword staging loads, and the register-staged one), on
+    128-row x 8-K-block tiles (two K-blocks of padding on most), and the
+    LDS-staged VALU kernel."""
     for k in ("IHE_CLOUD_KERNEL", "IHE_CLOUD_SHAPE", "IHE_CLOUD_NTW", "IHE_MFMAW_LDS"):
         monkeypatch.delenv(k, raising=False)
     if kernel == "registers":
         monkeypatch.setenv("IHE_MFMAW_LDS", "0")
+    elif kernel == "dword":
+        monkeypatch.setenv("IHE_MFMAW_LDS", "1")
     elif kernel == "staged":
         monkeypatch.setenv("IHE_CLOUD_KERNEL", kernel)
     elif kernel == "4,8":
