@@ -149,17 +149,25 @@ def forms_batched(eng, base, log_fn, seed=4):
         for band, ns, num_in, d_in, d_iv, d_out, d_ov in cohorts:
             eng.cloud_layout(band, ns, num_in, d_in, d_iv, d_out, d_ov)
 
-    os.environ["IHE_BATCH_PERSISTENT"] = "0"             # one launch per kernel family
-    try:
-        fam = eng.batch(cohorts)
-    finally:
-        os.environ.pop("IHE_BATCH_PERSISTENT", None)
+    def make_batch(mode):
+        os.environ["IHE_BATCH_PERSISTENT"] = mode
+        try:
+            return eng.batch(cohorts)
+        finally:
+            os.environ.pop("IHE_BATCH_PERSISTENT", None)
+
+    fam = make_batch("0")                                # one launch per kernel family
     fam_launches = fam.info()["num_launches"]
-    batch = eng.batch(cohorts)                           # one persistent launch
+    fam_graph = make_batch("0")                          # the same, replayed from a hipGraph
+    fam_graph.capture()
+    batch = make_batch("1")                              # one persistent launch
     info = batch.info()
     launches, persistent = info["num_launches"], bool(info["persistent"])
-    graph = eng.batch(cohorts)                           # the same, replayed from a hipGraph
+    graph = make_batch("1")                              # the same, replayed from a hipGraph
     graph.capture()
+    default = eng.batch(cohorts)                         # the library's default form
+    default_form = "one_launch" if default.info()["persistent"] else "per_family"
+    default.free()
     # each form's outputs against the per-cohort launches', then the forms
     # timed in turn, round after round (so that no form always runs on a
     # warmer chip), median of each
@@ -167,26 +175,26 @@ def forms_batched(eng, base, log_fn, seed=4):
     eng.sync()
     ref = sums()
     same = {}
-    for name, fn in (("family", fam.run), ("batch", batch.run), ("graph", graph.run)):
+    for name, fn in (("family", fam.run), ("family_graph", fam_graph.run), ("batch", batch.run),
+                     ("graph", graph.run)):
         for c in cohorts:
             c[5].fill_zero()
             c[6].fill_zero()
         fn()
         eng.sync()
         same[name] = sums() == ref
-    times = {"seq": [], "family": [], "batch": [], "graph": []}
+    times = {"seq": [], "family": [], "family_graph": [], "batch": [], "graph": []}
     for _ in range(5):
-        for name, fn in (("seq", sequential), ("family", fam.run), ("batch", batch.run),
-                         ("graph", graph.run)):
+        for name, fn in (("seq", sequential), ("family", fam.run), ("family_graph", fam_graph.run),
+                         ("batch", batch.run), ("graph", graph.run)):
             eng.sync()
             t0 = time.perf_counter()
             fn()
             eng.sync()
             times[name].append(time.perf_counter() - t0)
-    t_seq, t_fam, t_batch, t_graph = (float(np.median(times[k]))
-                                      for k in ("seq", "family", "batch", "graph"))
-    same_fam, same_batch, same_graph = same["family"], same["batch"], same["graph"]
-    for b in (fam, batch, graph):
+    t_seq, t_fam, t_fam_graph, t_batch, t_graph = (
+        float(np.median(times[k])) for k in ("seq", "family", "family_graph", "batch", "graph"))
+    for b in (fam, fam_graph, batch, graph):
         b.free()
     for c in cohorts:
         for d in c[3:]:
@@ -194,19 +202,22 @@ def forms_batched(eng, base, log_fn, seed=4):
     for b in bands.values():
         b.free()
     key.free()
-    if not (same_batch and same_graph and same_fam):
+    if not all(same.values()):
         raise SystemExit("batched outputs differ from the per-cohort launches")
     out = {"cohorts": 64, "imputed_target_snps_per_step": work, "kernel_launches": launches,
-           "persistent": persistent, "sequential_ms": round(t_seq * 1e3, 3),
+           "persistent": persistent, "default_form": default_form,
+           "sequential_ms": round(t_seq * 1e3, 3),
            "per_family_ms": round(t_fam * 1e3, 3), "per_family_launches": fam_launches,
+           "per_family_graph_ms": round(t_fam_graph * 1e3, 3),
            "batched_ms": round(t_batch * 1e3, 3), "graph_ms": round(t_graph * 1e3, 3),
            "sequential_value": round(work / t_seq, 1), "batched_value": round(work / t_batch, 1),
            "graph_value": round(work / t_graph, 1), "bit_exact": True,
            "tile_shapes": {str(n): list(v) for n, v in shapes.items()}}
     log_fn(f"batched form: 64 cohorts, sequential {t_seq * 1e3:.2f} ms, per family {t_fam * 1e3:.2f} "
-           f"ms ({fam_launches} launches), batch {t_batch * 1e3:.2f} ms ({launches} launch"
-           f"{'es' if launches != 1 else ''}{', persistent' if persistent else ''}), graph "
-           f"{t_graph * 1e3:.2f} ms; outputs identical")
+           f"ms ({fam_launches} launches; graph {t_fam_graph * 1e3:.2f} ms), one kernel "
+           f"{t_batch * 1e3:.2f} ms ({launches} launch{'es' if launches != 1 else ''}"
+           f"{', persistent' if persistent else ''}; graph {t_graph * 1e3:.2f} ms), default form "
+           f"{default_form}; outputs identical")
     return out
 
 

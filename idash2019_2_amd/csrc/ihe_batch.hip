@@ -13,7 +13,12 @@
 // own contiguous share of that list (shares cut by estimated time), so
 // consecutive blocks of one cohort meet in one L2. A batch with a family the
 // kernel has no body for (or with IHE_BATCH_PERSISTENT=0) runs one launch
-// per family instead, the variance blocks joining the last.
+// per family instead, the variance blocks joining the last. By default a
+// batch with 128-row families runs per family too: their own kernel stages
+// input words in LDS at three workgroups per CU, where the one kernel's
+// register budget (its largest body's) holds two workgroups per CU and its
+// 128-row bodies stage in registers; IHE_BATCH_PERSISTENT=1 asks for the one
+// kernel whenever it has a body for every family.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -195,11 +200,14 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
   }
   b->num_ventries = (uint32_t)vent.size();
   b->num_vblocks = vfirst.back();
-  // one persistent launch when every family has a body there
+  // one persistent launch when every family has a body there; by default
+  // only when every family is a one-K-block one
   const char *pe = std::getenv("IHE_BATCH_PERSISTENT");
-  b->persistent = !b->launches.empty() && !(pe && std::strcmp(pe, "0") == 0);
+  const bool ask_off = pe && std::strcmp(pe, "0") == 0, ask_on = pe && std::strcmp(pe, "1") == 0;
+  b->persistent = !b->launches.empty() && !ask_off;
   for (const ihe_batch::Launch &L : b->launches)
-    b->persistent = b->persistent && persist_body(L.shape, L.rot) >= 0;
+    b->persistent = b->persistent && persist_body(L.shape, L.rot) >= 0 &&
+                    (ask_on || L.shape.rb != kMfmaWideRb);
   if (b->persistent) {
     if (int rc = persist_grid(ctx->device, b->pgrid)) {
       delete b;

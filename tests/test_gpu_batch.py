@@ -96,7 +96,7 @@ def test_batch_matches_per_cohort_launches_and_oracle(engine, setup, mode, monke
     if mode == "families":
         monkeypatch.setenv("IHE_BATCH_PERSISTENT", "0")
     else:
-        monkeypatch.delenv("IHE_BATCH_PERSISTENT", raising=False)
+        monkeypatch.setenv("IHE_BATCH_PERSISTENT", "1")
         keys = [k for k in models if persistent_capable(bands[k].info())]
         rows = {bands[k].info()["tile_rows"] for k in keys}
         assert len(keys) >= 4 and 128 in rows and rows & {32, 64}, keys
@@ -206,14 +206,42 @@ def test_a_band_outlives_its_batch(engine, setup):
     band.free()
 
 
+def test_default_form(engine, setup, monkeypatch):
+    """Without IHE_BATCH_PERSISTENT a batch is one persistent launch when all
+    its families are one-K-block ones, and one launch per family when a
+    128-row family is among them."""
+    monkeypatch.delenv("IHE_BATCH_PERSISTENT", raising=False)
+    models, bands = setup
+    rows = {k: bands[k].info()["tile_rows"] for k in models}
+    kb = {k: bands[k].info()["tile_kblocks"] for k in models}
+    one = [k for k in models if rows[k] in (32, 64) and kb[k] == 1]
+    wide = [k for k in models if rows[k] == 128 and kb[k] <= 6]
+    assert one and wide
+    for keys, persistent in ((one[:2], 1), ([one[0], wide[0]], 0)):
+        cohorts, _ = make_cohorts(engine, models, bands, [1024, 100, 33], keys=keys)
+        batch = engine.batch(cohorts)
+        assert batch.info()["persistent"] == persistent
+        batch.run()
+        engine.sync()
+        for band, ns, num_in, d_in, d_var, d_out, d_ov in cohorts:
+            got, gv = d_out.numpy(), d_ov.numpy()
+            engine.cloud_layout(band, ns, num_in, d_in, d_var, d_out, d_ov)
+            assert (d_out.numpy() == got).all() and (d_ov.numpy() == gv).all()
+        batch.free()
+        for c in cohorts:
+            for d in c[3:]:
+                d.free()
+
+
 @pytest.mark.parametrize("kinds", ["one-K-block only", "128-row only", "one cohort"])
 @pytest.mark.parametrize("with_var", [True, False])
-def test_one_launch_edge_mixes(engine, setup, kinds, with_var):
+def test_one_launch_edge_mixes(engine, setup, kinds, with_var, monkeypatch):
     """The one-launch batch when one of its two parts is empty: only one-K-block
     families (the resident workgroups get variance blocks or nothing), only
     128-row families (no leading blocks), or a single cohort; with and without
     variances. Two runs in a row (the work counters start at zero again), each
     bit-exact against the per-cohort launches."""
+    monkeypatch.setenv("IHE_BATCH_PERSISTENT", "1")
     models, bands = setup
     rows = {k: bands[k].info()["tile_rows"] for k in models}
     kb = {k: bands[k].info()["tile_kblocks"] for k in models}

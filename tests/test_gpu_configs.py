@@ -212,8 +212,9 @@ def test_batched_workload_full_layout(engine, monkeypatch):
     to 64), one of 3 model sets, n in {5, 10, 20, 50}, all on the 16,184-tag /
     80,882-target layout; inputs encrypted on the device with a key per cohort.
     The bands take the tilings the batched bench tunes to (64 rows x 1 K-block
-    at n = 5, 128 rows x 2, 4, 6 K-blocks above), so the batch is one
-    persistent launch."""
+    at n = 5, 128 rows x 2, 4, 6 K-blocks above): with IHE_BATCH_PERSISTENT=1
+    the batch is one persistent launch, by default one launch per kernel
+    family; both equal the per-cohort launches."""
     base = synth.workload("full")
     rng = np.random.default_rng(4)
     models = synth.model_sets(base, (5, 10, 20, 50), 3, 4)
@@ -243,7 +244,9 @@ def test_batched_workload_full_layout(engine, monkeypatch):
         cohorts.append((bands[(s, n)], ns, num_in, d_in, d_iv, d_out, d_ov))
     packings = {1024 // ns for ns, _, _ in specs}
     assert len(packings) >= 8 and 1 in packings and max(packings) >= 16
+    monkeypatch.setenv("IHE_BATCH_PERSISTENT", "1")
     batch = engine.batch(cohorts)
+    monkeypatch.delenv("IHE_BATCH_PERSISTENT")
     assert batch.info()["persistent"] == 1 and batch.info()["num_launches"] == 1
     batch.run()
     engine.sync()
@@ -265,6 +268,16 @@ def test_batched_workload_full_layout(engine, monkeypatch):
         engine.sync()
         assert [(engine.checksum(co[5]), engine.checksum(co[6])) for co in cohorts] == sums
     batch.free()
+    # the default form: one launch per kernel family (128-row families present)
+    fam = engine.batch(cohorts)
+    assert fam.info()["persistent"] == 0 and fam.info()["num_launches"] >= 4
+    for co in cohorts:
+        co[5].fill_zero()
+        co[6].fill_zero()
+    fam.run()
+    engine.sync()
+    assert [(engine.checksum(co[5]), engine.checksum(co[6])) for co in cohorts] == sums
+    fam.free()
     # and every cohort equals its own per-cohort launch
     for co, want in zip(cohorts, sums):
         band, ns, num_in, d_in, d_iv, d_out, d_ov = co
