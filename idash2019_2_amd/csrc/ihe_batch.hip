@@ -37,6 +37,7 @@ struct ihe_batch {
     bool rot;
     uint32_t entry0, num_entries, first0, num_items;
     uint32_t item0;  // first tile in the tile -> entry table
+    bool runs4;      // rotated 128-row family whose region sizes are multiples of 4
   };
   std::vector<Launch> launches;
   bool persistent = false;
@@ -63,11 +64,14 @@ struct Keyed {
   uint32_t cohort;
   MfmaEntry e;
   uint32_t tiles;
+  // rotated 128-row tiles whose region size is a multiple of 4: their own
+  // family, launched with 16-byte staging loads
+  bool runs4;
 };
 
 bool same_family(const Keyed &a, const Keyed &b) {
   return a.shape.rb == b.shape.rb && a.shape.nkb == b.shape.nkb && a.shape.ntw == b.shape.ntw &&
-         a.rot == b.rot;
+         a.rot == b.rot && a.runs4 == b.runs4;
 }
 
 VarArgs var_args(const ihe_batch *b) {
@@ -100,7 +104,7 @@ void enqueue(const ihe_batch *b, hipStream_t s) {
     const ihe_batch::Launch &L = b->launches[i];
     launch_mfma(s, L.shape, L.rot, none, b->d_entries + L.entry0, b->d_first + L.first0,
                 L.num_entries, L.num_items, i + 1 == b->launches.size() ? V : VarArgs{},
-                b->d_item_entry + L.item0);
+                b->d_item_entry + L.item0, L.runs4);
   }
   if (b->launches.empty()) launch_var(s, V);
 }
@@ -159,6 +163,7 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
       E.out = c.d_out + (size_t)s.out_base * kCtWords;
       E.out_bytes = std::min(band->num_out - s.out_base, kSegCts) * kCtWords * 4u;
       E.tile_base = s.tile_begin;
+      k.runs4 = k.rot && k.shape.rb == kMfmaWideRb && E.rs % 4 == 0;
       if (k.tiles) keyed.push_back(k);
     }
     if (c.d_out_var) {
@@ -171,6 +176,7 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
   // kernel families (tile shape x rotation), cohorts in order within each
   std::stable_sort(keyed.begin(), keyed.end(), [](const Keyed &a, const Keyed &b) {
     if (a.rot != b.rot) return a.rot < b.rot;
+    if (a.runs4 != b.runs4) return a.runs4 < b.runs4;
     if (a.shape.nkb != b.shape.nkb) return a.shape.nkb < b.shape.nkb;
     if (a.shape.rb != b.shape.rb) return a.shape.rb < b.shape.rb;
     return a.shape.ntw < b.shape.ntw;
@@ -183,7 +189,7 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
   for (size_t i = 0; i < keyed.size();) {
     size_t j = i;
     ihe_batch::Launch L{keyed[i].shape, keyed[i].rot, (uint32_t)entries.size(), 0,
-                        (uint32_t)first.size(), 0, (uint32_t)item_entry.size()};
+                        (uint32_t)first.size(), 0, (uint32_t)item_entry.size(), keyed[i].runs4};
     uint32_t items = 0;
     for (; j < keyed.size() && same_family(keyed[i], keyed[j]); ++j) {
       entries.push_back(keyed[j].e);

@@ -163,9 +163,12 @@ void launch_persistent(hipStream_t stream, const PersistArgs &P, uint32_t grid);
 // variance blocks to the same grid: the step is one launch, and the variance
 // blocks fill the cloud blocks' drain tail.
 // item_entry (batches, optional): the entry of each of the num_items tiles,
-// in place of a search over `first`.
+// in place of a search over `first`. rot_runs4 (batches): every entry's
+// region size is a multiple of 4, so rotated columns keep their words in runs
+// of four (a single cohort's is read from `single`).
 void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const MfmaEntry &single,
                  const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
-                 uint32_t num_items, const VarArgs &var, const uint32_t *item_entry = nullptr);
+                 uint32_t num_items, const VarArgs &var, const uint32_t *item_entry = nullptr,
+                 bool rot_runs4 = false);
 
 }  // namespace ihe

@@ -598,8 +598,9 @@ void k_cloud_mfmaw(MfmaArgs A) {
 // barriers.
 // s_raw: [NKB * 32 columns][32 words]. With dword loads, load instruction q of
 // the tile writes columns 2q (lanes 0..31) and 2q + 1 (lanes 32..63); with
-// 16-byte loads (W16: unrotated only, where a column's 32 words are 128
-// consecutive bytes) it writes columns 8q..8q + 7, four words per lane.
+// 16-byte loads (W16, where a column's words come in runs of four: unrotated,
+// or rotated by multiples of 4) it writes columns 8q..8q + 7, four words per
+// lane.
 template <bool ROT, bool W16, int NTW, int NKB>
 __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, int4v *s_b,
                                                uint32_t *s_raw, uint32_t *s_cw, uint32_t *s_bias,
@@ -634,14 +635,26 @@ __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, 
   const uint32_t row_base = T.slot0 * kCtWords * 4u;
   const bool full = T.nrows == rows;
   typedef __attribute__((address_space(3))) void *lds_ptr;
-  static_assert(!(ROT && W16), "a rotated column's words are not 16-byte runs");
   static_assert((NKB * 4) % kWaves == 0, "16-byte staging loads split evenly over the waves");
+  // byte offset of word `word` (0..2047: a then b) of column word cw: a
+  // rotated column reads word i + m of its ciphertext for i + m < N, else
+  // word i + m - N (negated in the recoding)
+  auto word_off = [&](uint32_t cw, uint32_t word) {
+    if constexpr (!ROT) {
+      return cw + word * 4u;
+    } else {
+      const uint32_t tt = (word & (kN - 1)) + ((cw >> 2) & (kN - 1));
+      return (cw & ~0x1FFFu) + ((tt & (kN - 1)) << 2) + (word & kN) * 4u;
+    }
+  };
+  // W16 with ROT: every column's shift m = (c mod R) * RS is a multiple of 4
+  // (RS is), so the 16-byte runs stay whole and none wraps past N
   auto issue_raw = [&](uint32_t nt) {
     if constexpr (W16) {
 #pragma unroll
       for (int i = 0; i < NKB * 4 / kWaves; ++i) {
         const uint32_t q = (uint32_t)i * kWaves + wave;
-        const uint32_t off = s_cw[8 * q + lane / 8] + col(nt) * 128u + (lane % 8) * 16u;
+        const uint32_t off = word_off(s_cw[8 * q + lane / 8], col(nt) * 32 + (lane % 8) * 4);
         __builtin_amdgcn_raw_ptr_buffer_load_lds(in_r, (lds_ptr)(s_raw + q * 256), 16, off, 0, 0,
                                                  0);
       }
@@ -650,14 +663,7 @@ __device__ __forceinline__ void mfmaw_lds_body(const MfmaArgs &A, uint32_t blk, 
 #pragma unroll
     for (int i = 0; i < NKB * 4; ++i) {
       const uint32_t q = (uint32_t)i * kWaves + wave;
-      const uint32_t cw = s_cw[2 * q + h];
-      uint32_t off;
-      if constexpr (!ROT) {
-        off = cw + col(nt) * 128u + r * 4u;
-      } else {  // load_col's address of word col(nt) * 32 + r
-        const uint32_t word = col(nt) * 32 + r, tt = (word & (kN - 1)) + ((cw >> 2) & (kN - 1));
-        off = (cw & ~0x1FFFu) + ((tt & (kN - 1)) << 2) + (word & kN) * 4u;
-      }
+      const uint32_t off = word_off(s_cw[2 * q + h], col(nt) * 32 + r);
       __builtin_amdgcn_raw_ptr_buffer_load_lds(in_r, (lds_ptr)(s_raw + q * 64), 4, off, 0, 0, 0);
     }
   };
@@ -973,14 +979,14 @@ void launch_kw_lds(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
 // tiles of nkb K-blocks run on the kernel with the next even NKB (padding
 // K-blocks carry zero weights). IHE_MFMAW_LDS: 0 = register-staged kernel,
 // 1 = LDS-staged with dword staging loads, else LDS-staged with 16-byte
-// staging loads where the layout is unrotated.
+// staging loads where a column's words come in runs of four.
 template <bool ROT, int NTW>
-void launch_kw(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
+void launch_kw(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A, bool runs4) {
   const dim3 block(64 * kWaves);
   const char *e = std::getenv("IHE_MFMAW_LDS");
   const char mode = e && (e[0] == '0' || e[0] == '1') ? e[0] : '2';
   if (mode != '0' && nkb <= 6) {
-    if (!ROT && mode == '2') launch_kw_lds<false, true, NTW>(s, nkb, grid, A);
+    if (mode == '2' && runs4) launch_kw_lds<ROT, true, NTW>(s, nkb, grid, A);
     else launch_kw_lds<ROT, false, NTW>(s, nkb, grid, A);
     return;
   }
@@ -993,11 +999,11 @@ void launch_kw(hipStream_t s, int nkb, dim3 grid, const MfmaArgs &A) {
 }
 
 template <bool ROT>
-void launch_rot(hipStream_t s, const MfmaShape &sh, dim3 grid, const MfmaArgs &A) {
+void launch_rot(hipStream_t s, const MfmaShape &sh, dim3 grid, const MfmaArgs &A, bool runs4) {
   if (sh.rb == kMfmaWideRb) {
-    if (sh.ntw == 32) launch_kw<ROT, 32>(s, sh.nkb, grid, A);
-    else if (sh.ntw == 16) launch_kw<ROT, 16>(s, sh.nkb, grid, A);
-    else launch_kw<ROT, 8>(s, sh.nkb, grid, A);
+    if (sh.ntw == 32) launch_kw<ROT, 32>(s, sh.nkb, grid, A, runs4);
+    else if (sh.ntw == 16) launch_kw<ROT, 16>(s, sh.nkb, grid, A, runs4);
+    else launch_kw<ROT, 8>(s, sh.nkb, grid, A, runs4);
   } else if (sh.nkb == 1) {
     if (sh.ntw == 16) launch_k1<ROT, 16>(s, sh.rb, grid, A);
     else if (sh.ntw == 8) launch_k1<ROT, 8>(s, sh.rb, grid, A);
@@ -1013,13 +1019,14 @@ void launch_rot(hipStream_t s, const MfmaShape &sh, dim3 grid, const MfmaArgs &A
 
 void launch_mfma(hipStream_t stream, const MfmaShape &shape, bool rot, const MfmaEntry &single,
                  const MfmaEntry *entries, const uint32_t *first, uint32_t num_entries,
-                 uint32_t num_items, const VarArgs &var, const uint32_t *item_entry) {
+                 uint32_t num_items, const VarArgs &var, const uint32_t *item_entry,
+                 bool rot_runs4) {
   if (num_items == 0) return launch_var(stream, var);
   const uint32_t cloud_blocks = num_items * mfma_wg_per_tile(shape);
   const MfmaArgs A = {single, entries, first, item_entry, num_entries, cloud_blocks, var};
   const dim3 grid(cloud_blocks + var.num_blocks);
-  if (rot) launch_rot<true>(stream, shape, grid, A);
-  else launch_rot<false>(stream, shape, grid, A);
+  if (rot) launch_rot<true>(stream, shape, grid, A, entries ? rot_runs4 : single.rs % 4 == 0);
+  else launch_rot<false>(stream, shape, grid, A, true);
 }
 
 int persist_body(const MfmaShape &shape, bool rot) {

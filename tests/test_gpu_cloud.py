@@ -13,9 +13,10 @@ pytestmark = pytest.mark.gpu
 # 64-row single-K-block tiles (the default at 5 neighbours), 32-row tiles,
 # and 2 or 3 K-blocks (forcing padded K-blocks on narrow windows); NTW = 32-word
 # column tiles per wave. 128-row tiles ("w") run the LDS-staged kernel up to 6
-# K-blocks (16-byte staging loads where unrotated; IHE_MFMAW_LDS=1 forces dword
-# loads) and the register-staged one above (IHE_MFMAW_LDS=0 forces it). Then
-# the LDS-staged dot2 VALU and direct dot2 VALU.
+# K-blocks (16-byte staging loads where unrotated or rotated with RS % 4 == 0,
+# as at NS = 128 and 512; IHE_MFMAW_LDS=1 forces dword loads) and the
+# register-staged one above (IHE_MFMAW_LDS=0 forces it). Then the LDS-staged
+# dot2 VALU and direct dot2 VALU.
 KERNELS = {
     "mfma": {},
     "mfma-rb1": {"IHE_CLOUD_SHAPE": "1,1"},
