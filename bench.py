@@ -361,6 +361,69 @@ def cli_pipeline(workload, log_fn, workdir=None):
     return {"workload": workload, "figures": figures, "file_bytes": sizes}
 
 
+def staging_ab(eng, layouts=(1024, 512, 200, 100), rounds=3, launches=20) -> str:
+    """The 128-row cloud kernel's staging forms side by side on the full SNP
+    layout (16,184 tags, 80,882 targets): input words staged in registers
+    (IHE_MFMAW_LDS=0), staged in LDS by dword loads (1) and by 16-byte loads
+    where a column's words come in runs of four (the default), on 128 x 2, 4 and
+    6 K-block tiles at 10, 20 and 50 neighbours, for several sample counts.
+    Forms alternate launch by launch group, `rounds` times; each figure is the
+    best round's mean over `launches` launches (HIP events), and every form's
+    outputs must agree."""
+    base = synth.workload("full")
+    models = synth.model_sets(base, (10, 20, 50), 1, 4)
+    forms = (("0", "registers"), ("1", "LDS, dword"), ("2", "LDS, 16-byte"))
+    lines = ["# The 128-row cloud kernel's staging forms (IHE_MFMAW_LDS), full SNP layout,",
+             f"# best of {rounds} interleaved rounds of {launches} launches; outputs identical",
+             "#", "# Command (from the repository root):", "#   python3 bench.py --staging-ab",
+             f"# source_hash of the library sources measured: {source_hash()}", "#",
+             f"{'NS':>5} {'n':>3} {'tile':>8}  " + "  ".join(f"{name:>13}" for _, name in forms)]
+    key = eng.key(base.key)
+    for ns in layouts:
+        R = 1024 // ns
+        num_in = (3 * base.num_tags + R - 1) // R
+        cols = np.random.default_rng(ns).choice(1024, ns, replace=False)
+        d_g = eng.to_device(np.ascontiguousarray(base.geno[:, cols]))
+        d_in, d_iv = eng.empty((num_in, 2, 1024), np.int32), eng.empty((num_in,), np.float64)
+        eng.encrypt(key, ns, d_g, 11, d_in, d_iv)
+        d_g.free()
+        d_out = eng.empty((base.num_outputs, 2, 1024), np.int32)
+        d_ov = eng.empty((base.num_outputs,), np.float64)
+        for n, shape in ((10, "4,2"), (20, "4,4"), (50, "4,6")):
+            os.environ["IHE_CLOUD_SHAPE"] = shape
+            try:
+                band = eng.band(1024, base.num_in_cts, *models[(0, n)].model_csr())
+            finally:
+                os.environ.pop("IHE_CLOUD_SHAPE", None)
+            best, sums = {}, {}
+            try:
+                for _rnd in range(rounds):
+                    for mode, _name in forms:
+                        os.environ["IHE_MFMAW_LDS"] = mode
+                        for _ in range(3):
+                            eng.cloud_layout(band, ns, num_in, d_in, d_iv, d_out, d_ov)
+                        eng.sync()
+                        eng.timing(True)
+                        eng.timing_reset()
+                        for _ in range(launches):
+                            eng.cloud_layout(band, ns, num_in, d_in, d_iv, d_out, d_ov)
+                        ms, cnt = eng.timing_read(_abi.TIMER_CLOUD)
+                        eng.timing(False)
+                        best[mode] = min(best.get(mode, float("inf")), ms / cnt)
+                        sums[mode] = (eng.checksum(d_out), eng.checksum(d_ov))
+            finally:
+                os.environ.pop("IHE_MFMAW_LDS", None)
+            band.free()
+            if len(set(sums.values())) != 1:
+                raise SystemExit(f"staging forms disagree at NS={ns}, n={n}")
+            lines.append(f"{ns:>5} {n:>3} {'128 x ' + shape[2]:>8}  " +
+                         "  ".join(f"{best[m]:>10.4f} ms" for m, _ in forms))
+        for d in (d_in, d_iv, d_out, d_ov):
+            d.free()
+    key.free()
+    return "\n".join(lines) + "\n"
+
+
 def hbm_probes(eng) -> str:
     """What HBM sustains on this device for the streaming probe's access
     patterns (contiguous 16-byte loads and stores, 1 KiB per wave instruction
@@ -547,6 +610,9 @@ def main(argv=None):
     ap.add_argument("--probes", action="store_true",
                     help="HBM access patterns and cache policies of the streaming probe (a text "
                          "table on stdout)")
+    ap.add_argument("--staging-ab", action="store_true",
+                    help="the 128-row cloud kernel's staging forms side by side on the full "
+                         "layout (a text table on stdout)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (a kernel of known "
                          "traffic for scaling HBM counters) and exit")
@@ -581,6 +647,11 @@ def main(argv=None):
     if args.probes:
         with Engine(0) as eng:
             sys.stdout.write(hbm_probes(eng))
+        return 0
+
+    if args.staging_ab:
+        with Engine(0) as eng:
+            sys.stdout.write(staging_ab(eng))
         return 0
 
     if args.calibrate_copy:
