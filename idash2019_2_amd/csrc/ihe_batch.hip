@@ -54,6 +54,11 @@ struct ihe_batch {
   uint32_t *d_vfirst = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // per-family launches alternate between the context's stream and `side`,
+  // forked and joined by events, so that one family's drain overlaps the
+  // next one's start (null: every launch on the context's stream)
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace {
@@ -95,16 +100,32 @@ double block_words(const MfmaShape &shape) {
   return (double)kCtWords * kMfmaRows * shape.rb / persist_wg_per_tile(shape);
 }
 
+void release_side(ihe_batch *b) {
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
+  if (b->side) (void)hipStreamDestroy(b->side);
+  b->fork = b->join = nullptr;
+  b->side = nullptr;
+}
+
 void enqueue(const ihe_batch *b, hipStream_t s) {
   if (b->persistent) return launch_persistent(s, b->P, b->pgrid);
   MfmaEntry none = {};
   const VarArgs V = var_args(b);
+  if (b->side) {
+    (void)hipEventRecord(b->fork, s);
+    (void)hipStreamWaitEvent(b->side, b->fork, 0);
+  }
   // the variance blocks join the last family's launch
   for (size_t i = 0; i < b->launches.size(); ++i) {
     const ihe_batch::Launch &L = b->launches[i];
-    launch_mfma(s, L.shape, L.rot, none, b->d_entries + L.entry0, b->d_first + L.first0,
-                L.num_entries, L.num_items, i + 1 == b->launches.size() ? V : VarArgs{},
-                b->d_item_entry + L.item0, L.runs4);
+    launch_mfma(b->side && i % 2 ? b->side : s, L.shape, L.rot, none, b->d_entries + L.entry0,
+                b->d_first + L.first0, L.num_entries, L.num_items,
+                i + 1 == b->launches.size() ? V : VarArgs{}, b->d_item_entry + L.item0, L.runs4);
+  }
+  if (b->side) {
+    (void)hipEventRecord(b->join, b->side);
+    (void)hipStreamWaitEvent(s, b->join, 0);
   }
   if (b->launches.empty()) launch_var(s, V);
 }
@@ -322,6 +343,16 @@ int ihe_batch_create(ihe_ctx *ctx, const ihe_cohort *cohorts, uint32_t num_cohor
     if (std::find(b->bands.begin(), b->bands.end(), cohorts[i].band) == b->bands.end())
       b->bands.push_back(cohorts[i].band);
   for (const ihe_band *band : b->bands) band->users++;
+  // a second stream for per-family launches (IHE_BATCH_STREAMS=1: one stream)
+  const char *se = std::getenv("IHE_BATCH_STREAMS");
+  if (!b->persistent && b->launches.size() > 1 && !(se && std::strcmp(se, "1") == 0)) {
+    if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      release_side(b);
+    }
+  }
   *out = b;
   return IHE_OK;
 }
@@ -370,6 +401,7 @@ int ihe_batch_destroy(ihe_batch *b) {
   (void)ihe_sync(b->ctx);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  release_side(b);
   (void)ihe_free(b->ctx, b->dmem);
   for (const ihe_band *band : b->bands) band->users--;
   delete b;

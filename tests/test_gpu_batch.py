@@ -85,16 +85,20 @@ def make_cohorts(engine, models, bands, sample_counts, with_var=True, keys=None)
     return cohorts, host
 
 
-@pytest.mark.parametrize("mode", ["persistent", "families"])
+@pytest.mark.parametrize("mode", ["persistent", "families", "families-one-stream"])
 def test_batch_matches_per_cohort_launches_and_oracle(engine, setup, mode, monkeypatch):
     """Persistent: cohorts on every band whose tiling the persistent kernel
     runs (both row-block kinds, rotated and not), one launch. Families: every
-    band, including per-wave multi-K-block tilings, one launch per family."""
+    band, including per-wave multi-K-block tilings, one launch per family,
+    alternating over two streams (or all on one, IHE_BATCH_STREAMS=1)."""
     models, bands = setup
     counts = [1024, 700, 341, 200, 128, 100, 64, 33, 16, 1024, 512, 77]
     keys = None
-    if mode == "families":
+    monkeypatch.delenv("IHE_BATCH_STREAMS", raising=False)
+    if mode.startswith("families"):
         monkeypatch.setenv("IHE_BATCH_PERSISTENT", "0")
+        if mode == "families-one-stream":
+            monkeypatch.setenv("IHE_BATCH_STREAMS", "1")
     else:
         monkeypatch.setenv("IHE_BATCH_PERSISTENT", "1")
         keys = [k for k in models if persistent_capable(bands[k].info())]
